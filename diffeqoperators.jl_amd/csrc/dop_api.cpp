@@ -1,0 +1,353 @@
+// dop_api.cpp -- the C ABI of libdop.so (include/dop.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dop.h"
+#include "dop_build.hpp"
+#include "dop_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define DOP_HIP(x)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(DOP_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+int device_grid(int per_cu)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256 * per_cu;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return cus * per_cu;
+}
+
+}  // namespace
+
+struct dop_plan {
+    dop::PlanDev host{};          // table pointers are device pointers
+    dop::PlanDev* dev = nullptr;  // device copy of host
+    std::vector<void*> bufs;
+    int64_t in_elems = 0, out_elems = 0;
+    int device = 0;
+    int variant = 0;
+    ~dop_plan()
+    {
+        for (void* b : bufs) (void)hipFree(b);
+        if (dev) (void)hipFree(dev);
+    }
+};
+
+struct dop_batch {
+    dop::BatchItemDev* items = nullptr;
+    int n = 0;
+    int64_t nchunks = 0;
+    int storage = 0;
+    int grid = 0;
+    ~dop_batch()
+    {
+        if (items) (void)hipFree(items);
+    }
+};
+
+struct dop_rk4 {
+    dop::RkItemDev* items = nullptr;
+    int n = 0;
+    int64_t nchunks = 0;
+    int storage = 0;
+    int grid = 0;
+    std::vector<void*> bufs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~dop_rk4()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* b : bufs) (void)hipFree(b);
+        if (items) (void)hipFree(items);
+    }
+};
+
+namespace {
+
+template <typename E>
+int upload(dop_plan* p, const std::vector<E>& v, const E** out)
+{
+    *out = nullptr;
+    if (v.empty()) return DOP_OK;
+    void* d = nullptr;
+    const size_t bytes = v.size() * sizeof(E);
+    if (hipMalloc(&d, bytes) != hipSuccess) return fail(DOP_ENOMEM, "hipMalloc of plan tables failed");
+    p->bufs.push_back(d);
+    DOP_HIP(hipMemcpy(d, v.data(), bytes, hipMemcpyHostToDevice));
+    *out = (const E*)d;
+    return DOP_OK;
+}
+
+size_t storage_bytes(int storage)
+{
+    return storage == DOP_F64 ? 8 : storage == DOP_F32 ? 4 : 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dop_version(void) { return DOP_ABI_VERSION; }
+
+const char* dop_last_error(void) { return g_err.c_str(); }
+
+int dop_plan_create(const dop_desc* desc, dop_plan** out)
+{
+    if (!desc || !out) return fail(DOP_EINVAL, "null argument");
+    *out = nullptr;
+    dop::HostPlan H;
+    std::string err;
+    const int rc = dop::build_host_plan(*desc, &H, &err);
+    if (rc != DOP_OK) return fail(rc, err);
+    dop_plan* p = new dop_plan();
+    p->host = H.meta;
+    p->in_elems = H.in_elems;
+    p->out_elems = H.out_elems;
+    (void)hipGetDevice(&p->device);
+    int r = DOP_OK;
+    for (int a = 0; a < dop::kMaxDim && r == DOP_OK; ++a) {
+        dop::AxisDev& X = p->host.ax[a];
+        const dop::AxisTables& T = H.axis[a];
+        if ((r = upload(p, T.w64, &X.w64)) != DOP_OK) break;
+        if ((r = upload(p, T.w32, &X.w32)) != DOP_OK) break;
+        if ((r = upload(p, T.bJ0, &X.bJ0)) != DOP_OK) break;
+        if ((r = upload(p, T.bn, &X.bn)) != DOP_OK) break;
+        if ((r = upload(p, T.bw, &X.bw)) != DOP_OK) break;
+    }
+    if (r == DOP_OK) {
+        if (hipMalloc((void**)&p->dev, sizeof(dop::PlanDev)) != hipSuccess) {
+            r = fail(DOP_ENOMEM, "hipMalloc of plan failed");
+        } else if (hipMemcpy(p->dev, &p->host, sizeof(dop::PlanDev), hipMemcpyHostToDevice) !=
+                   hipSuccess) {
+            r = fail(DOP_EHIP, "upload of plan failed");
+        }
+    }
+    if (r != DOP_OK) {
+        delete p;
+        return r;
+    }
+    *out = p;
+    return DOP_OK;
+}
+
+void dop_plan_destroy(dop_plan* plan) { delete plan; }
+
+int64_t dop_plan_input_elems(const dop_plan* plan) { return plan ? plan->in_elems : -1; }
+int64_t dop_plan_output_elems(const dop_plan* plan) { return plan ? plan->out_elems : -1; }
+
+int dop_apply(const dop_plan* plan, void* du, const void* u, int accumulate, void* stream)
+{
+    if (!plan || !du || !u) return fail(DOP_EINVAL, "null plan or buffer");
+    if (du == u) return fail(DOP_EINVAL, "du must not alias u");
+    DOP_HIP(dop::launch_apply(plan->host, plan->dev, du, u, accumulate ? 1 : 0, plan->variant,
+                              (hipStream_t)stream));
+    return DOP_OK;
+}
+
+int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
+{
+    if (!plan) return fail(DOP_EINVAL, "null plan");
+    const int nv = dop::apply_variants(plan->host);
+    if (nv <= 1) {
+        plan->variant = 0;
+        if (choice) *choice = 0;
+        return DOP_OK;
+    }
+    const size_t sb = storage_bytes(plan->host.storage);
+    void *du = nullptr, *u = nullptr;
+    if (hipMalloc(&du, plan->out_elems * sb) != hipSuccess) return fail(DOP_ENOMEM, "autotune alloc");
+    if (hipMalloc(&u, plan->in_elems * sb) != hipSuccess) {
+        (void)hipFree(du);
+        return fail(DOP_ENOMEM, "autotune alloc");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipMemsetAsync(u, 0, plan->in_elems * sb, s);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float best = 1e30f;
+    int best_v = 0;
+    for (int v = 0; v < nv; ++v) {
+        float tsum = 0.0f;
+        bool ok = true;
+        for (int rep = 0; rep < 4 && ok; ++rep) {
+            (void)hipEventRecord(e0, s);
+            ok = dop::launch_apply(plan->host, plan->dev, du, u, 0, v, s) == hipSuccess;
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) tsum += ms;
+        }
+        if (ok && tsum < best) {
+            best = tsum;
+            best_v = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(du);
+    (void)hipFree(u);
+    plan->variant = best_v;
+    if (choice) *choice = best_v;
+    return DOP_OK;
+}
+
+int dop_batch_create(int n, const dop_plan* const* plans, void* const* du, const void* const* u,
+                     dop_batch** out)
+{
+    if (!out || n < 1 || !plans || !du || !u) return fail(DOP_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    std::vector<dop::BatchItemDev> items(n);
+    int64_t c = 0;
+    const int storage = plans[0] ? plans[0]->host.storage : 0;
+    for (int i = 0; i < n; ++i) {
+        if (!plans[i] || !du[i] || !u[i]) return fail(DOP_EINVAL, "null plan or buffer in batch");
+        if (plans[i]->host.storage != storage) return fail(DOP_EINVAL, "mixed storage types in batch");
+        if (du[i] == u[i]) return fail(DOP_EINVAL, "du must not alias u");
+        items[i].plan = plans[i]->dev;
+        items[i].du = du[i];
+        items[i].u = u[i];
+        items[i].total = plans[i]->host.total;
+        items[i].chunk0 = c;
+        c += (plans[i]->host.total + dop::kChunkPts - 1) / dop::kChunkPts;
+    }
+    dop_batch* b = new dop_batch();
+    b->n = n;
+    b->nchunks = c;
+    b->storage = storage;
+    b->grid = device_grid(8);
+    if (hipMalloc((void**)&b->items, sizeof(dop::BatchItemDev) * n) != hipSuccess) {
+        delete b;
+        return fail(DOP_ENOMEM, "hipMalloc of batch failed");
+    }
+    if (hipMemcpy(b->items, items.data(), sizeof(dop::BatchItemDev) * n, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        delete b;
+        return fail(DOP_EHIP, "upload of batch failed");
+    }
+    *out = b;
+    return DOP_OK;
+}
+
+int dop_batch_apply(const dop_batch* batch, int accumulate, void* stream)
+{
+    if (!batch) return fail(DOP_EINVAL, "null batch");
+    DOP_HIP(dop::launch_batch(batch->storage, batch->items, batch->n, batch->nchunks,
+                              accumulate ? 1 : 0, batch->grid, (hipStream_t)stream));
+    return DOP_OK;
+}
+
+void dop_batch_destroy(dop_batch* batch) { delete batch; }
+
+int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, const double* dt,
+                   void* const* u, void* stream, dop_rk4** out)
+{
+    if (!out || n < 1 || !plans || !scale || !dt || !u) return fail(DOP_EINVAL, "bad rk4 arguments");
+    *out = nullptr;
+    const int storage = plans[0] ? plans[0]->host.storage : 0;
+    if (storage != DOP_F64 && storage != DOP_F32)
+        return fail(DOP_EUNSUPPORTED, "rk4 needs f32 or f64 storage");
+    dop_rk4* rk = new dop_rk4();
+    rk->n = n;
+    rk->storage = storage;
+    rk->grid = device_grid(8);
+    std::vector<dop::RkItemDev> items(n);
+    int64_t c = 0;
+    const size_t sb = storage_bytes(storage);
+    for (int i = 0; i < n; ++i) {
+        const dop_plan* p = plans[i];
+        if (!p || !u[i] || p->host.storage != storage || !p->host.ghost) {
+            delete rk;
+            return fail(DOP_EINVAL, "rk4: plans must be Q*u plans of one storage type");
+        }
+        dop::RkItemDev& R = items[i];
+        R.plan = p->dev;
+        R.u = u[i];
+        R.scale = scale[i];
+        R.dt = dt[i];
+        R.total = p->host.total;
+        R.chunk0 = c;
+        c += (R.total + dop::kChunkPts - 1) / dop::kChunkPts;
+        void* bufs[3] = {nullptr, nullptr, nullptr};
+        for (int k = 0; k < 3; ++k) {
+            if (hipMalloc(&bufs[k], R.total * sb) != hipSuccess) {
+                delete rk;
+                return fail(DOP_ENOMEM, "rk4: stage buffer allocation failed");
+            }
+            rk->bufs.push_back(bufs[k]);
+        }
+        R.s1 = bufs[0];
+        R.s2 = bufs[1];
+        R.acc = bufs[2];
+    }
+    rk->nchunks = c;
+    if (hipMalloc((void**)&rk->items, sizeof(dop::RkItemDev) * n) != hipSuccess) {
+        delete rk;
+        return fail(DOP_ENOMEM, "rk4: item table allocation failed");
+    }
+    if (hipMemcpy(rk->items, items.data(), sizeof(dop::RkItemDev) * n, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        delete rk;
+        return fail(DOP_EHIP, "rk4: item upload failed");
+    }
+    // capture one step (four stage launches) into a graph
+    hipStream_t cs = nullptr;
+    int r = DOP_OK;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        delete rk;
+        return fail(DOP_EHIP, "rk4: stream creation failed");
+    }
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        r = fail(DOP_EHIP, "rk4: begin capture failed");
+    } else {
+        for (int st = 0; st < 4; ++st)
+            if (dop::launch_rk4_stage(storage, rk->items, n, rk->nchunks, st, rk->grid, cs) !=
+                hipSuccess)
+                r = fail(DOP_EHIP, "rk4: stage launch during capture failed");
+        if (hipStreamEndCapture(cs, &rk->graph) != hipSuccess && r == DOP_OK)
+            r = fail(DOP_EHIP, "rk4: end capture failed");
+        if (r == DOP_OK && hipGraphInstantiate(&rk->exec, rk->graph, nullptr, nullptr, 0) != hipSuccess)
+            r = fail(DOP_EHIP, "rk4: graph instantiation failed");
+    }
+    (void)hipStreamDestroy(cs);
+    (void)stream;
+    if (r != DOP_OK) {
+        delete rk;
+        return r;
+    }
+    *out = rk;
+    return DOP_OK;
+}
+
+int dop_rk4_step(dop_rk4* rk, int nsteps, void* stream)
+{
+    if (!rk || nsteps < 0) return fail(DOP_EINVAL, "bad rk4 step arguments");
+    for (int i = 0; i < nsteps; ++i) DOP_HIP(hipGraphLaunch(rk->exec, (hipStream_t)stream));
+    return DOP_OK;
+}
+
+void dop_rk4_destroy(dop_rk4* rk) { delete rk; }
+
+}  // extern "C"

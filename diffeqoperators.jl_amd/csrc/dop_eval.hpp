@@ -1,0 +1,185 @@
+// dop_eval.hpp -- storage conversions, accumulators and the per-point evaluation of a
+// plan (the generic path: any ndim, any row program).  __host__ __device__ so that the
+// index arithmetic can be checked on the CPU against the FP64 oracle.
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#include "dop_plan.hpp"
+
+namespace dop {
+
+// ---------------------------------------------------------------------------------------
+// storage types: double, float, and bf16 held as uint16_t
+struct bf16 {
+    uint16_t bits;
+};
+
+DOP_HD inline float bf16_to_f32(uint16_t b)
+{
+    return __builtin_bit_cast(float, (uint32_t)b << 16);
+}
+
+// round to nearest even; NaN stays NaN (quiet)
+DOP_HD inline uint16_t f32_to_bf16(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <typename T>
+struct Store;
+template <>
+struct Store<double> {
+    using C = double;
+    DOP_HD static double ld(const double* p, int64_t i) { return p[i]; }
+    DOP_HD static void st(double* p, int64_t i, double v) { p[i] = v; }
+};
+template <>
+struct Store<float> {
+    using C = float;
+    DOP_HD static float ld(const float* p, int64_t i) { return p[i]; }
+    DOP_HD static void st(float* p, int64_t i, double v) { p[i] = (float)v; }
+};
+template <>
+struct Store<bf16> {
+    using C = float;
+    DOP_HD static float ld(const bf16* p, int64_t i) { return bf16_to_f32(p[i].bits); }
+    DOP_HD static void st(bf16* p, int64_t i, double v) { p[i].bits = f32_to_bf16((float)v); }
+};
+
+// ---------------------------------------------------------------------------------------
+// accumulators
+template <int ACC>
+struct Acc;
+
+template <>
+struct Acc<kAccF64> {
+    using W = double;   // weight type of the regular rows
+    double s = 0.0;
+    DOP_HD void add(double w, double x) { s = fma(w, x, s); }
+    DOP_HD void add_sum(double v) { s += v; }
+    DOP_HD double value() const { return s; }
+    DOP_HD static const double* table(const AxisDev& X) { return X.w64; }
+};
+
+template <>
+struct Acc<kAccF32> {
+    using W = float;
+    float s = 0.0f;
+    DOP_HD void add(float w, float x) { s = fmaf(w, x, s); }
+    DOP_HD void add_sum(double v) { s += (float)v; }
+    DOP_HD double value() const { return s; }
+    DOP_HD static const float* table(const AxisDev& X) { return X.w32; }
+};
+
+// Compensated dot product in f32 (Ogita, Rump & Oishi 2005, "Dot2"): the product error
+// comes from one FMA (TwoProduct), the addition error from Knuth's TwoSum.  Contraction
+// is switched off so that the error terms are formed exactly as written.
+template <>
+struct Acc<kAccF32Kahan> {
+    using W = float;
+    float s = 0.0f, c = 0.0f;
+    DOP_HD void two_sum_add(float p)
+    {
+#pragma clang fp contract(off)
+        const float t = s + p;
+        const float z = t - s;
+        const float e = (s - (t - z)) + (p - z);
+        s = t;
+        c = c + e;
+    }
+    DOP_HD void add(float w, float x)
+    {
+#pragma clang fp contract(off)
+        const float p = w * x;
+        const float pe = fmaf(w, x, -p);
+        two_sum_add(p);
+        c = c + pe;
+    }
+    DOP_HD void add_sum(double v)
+    {
+#pragma clang fp contract(off)
+        const float hi = (float)v;
+        const float lo = (float)(v - (double)hi);
+        two_sum_add(hi);
+        c = c + lo;
+    }
+    DOP_HD double value() const
+    {
+#pragma clang fp contract(off)
+        return (double)s + (double)c;
+    }
+    DOP_HD static const float* table(const AxisDev& X) { return X.w32; }
+};
+
+// ---------------------------------------------------------------------------------------
+// ghost values of one line: a . u[0:nl] + b (bc_operators.jl:188-191)
+template <typename T>
+DOP_HD inline double ghost_left(const AxisDev& X, const T* src, int64_t base, int64_t S)
+{
+    double g = 0.0;
+    for (int k = 0; k < X.nl; ++k) g = fma(X.al[k], (double)Store<T>::ld(src, base + k * S), g);
+    return g + X.bl;
+}
+
+template <typename T>
+DOP_HD inline double ghost_right(const AxisDev& X, const T* src, int64_t base, int64_t S)
+{
+    double g = 0.0;
+    const int64_t o = (int64_t)X.n - X.nr;
+    for (int k = 0; k < X.nr; ++k)
+        g = fma(X.ar[k], (double)Store<T>::ld(src, base + (o + k) * S), g);
+    return g + X.br;
+}
+
+// FP64 value of a boundary row (always evaluated in double)
+template <typename T>
+DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* src, int r,
+                                  int64_t base, int64_t S)
+{
+    const int b = r < X.rlo ? r : X.rlo + (r - X.rhi);
+    const int J0 = X.bJ0[b];
+    const int nt = X.bn[b];
+    const double* w = X.bw + (int64_t)b * X.KB;
+    const int joff = P.ghost ? -1 : 0;
+    double s = 0.0;
+    for (int k = 0; k < nt; ++k) {
+        const int J = J0 + k;
+        double v;
+        if (P.ghost && J == 0) v = ghost_left<T>(X, src, base, S);
+        else if (P.ghost && J == X.n + 1) v = ghost_right<T>(X, src, base, S);
+        else v = (double)Store<T>::ld(src, base + (int64_t)(J + joff) * S);
+        s = fma(w[k], v, s);
+    }
+    return s;
+}
+
+// du value at multi-index idx (without the accumulate term)
+template <typename T, int ACC>
+DOP_HD inline double eval_point(const PlanDev& P, const T* src, const int64_t* idx)
+{
+    Acc<ACC> acc;
+    for (int a = 0; a < P.ndim; ++a) {
+        const AxisDev& X = P.ax[a];
+        if (!X.active) continue;
+        const int r = (int)idx[a];
+        const int64_t S = P.ssrc[a];
+        int64_t base = P.tshift[a];
+        for (int b = 0; b < P.ndim; ++b)
+            if (b != a) base += idx[b] * P.ssrc[b];
+        if (r >= X.rlo && r < X.rhi) {
+            const auto* w = Acc<ACC>::table(X) + (X.uniform ? 0 : (int64_t)r * X.K);
+            const int64_t p0 = base + (int64_t)(r + X.dmin + (P.ghost ? 0 : 1)) * S;
+            for (int k = 0; k < X.K; ++k)
+                acc.add(w[k], (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
+        } else {
+            acc.add_sum(boundary_row<T>(P, X, src, r, base, S));
+        }
+    }
+    return acc.value();
+}
+
+}  // namespace dop

@@ -1,0 +1,38 @@
+"""diffeqoperators_amd -- DiffEqOperators.jl's operator-application path on MI355X.
+
+The constructors and composition rules mirror the Julia package (CenteredDifference,
+UpwindDifference, RobinBC / DirichletBC / NeumannBC, compose, L*Q, L1+L2, mul!); applying
+an operator runs hand-written gfx950 HIP kernels in libdop.so (C ABI: include/dop.h).
+
+    import torch
+    from diffeqoperators_amd import CenteredDifference, Dirichlet0BC
+    Δ = CenteredDifference(2, 2, h, 100)
+    bc = Dirichlet0BC(float)
+    du = Δ * bc * u          # u: torch.float64 tensor on the GPU
+
+Arrays are C-ordered tensors whose shape is the Julia size reversed: Julia axis N is
+tensor dimension -N.
+"""
+from . import _abi
+from .apply import Plan, julia_size, mul_, plans_for
+from .batch import Batch, RK4
+from .bcs import (AffineBC, ComposedMultiDimBC, Dirichlet0BC, DirichletBC, GeneralBC,
+                  MultiDimBC, MultiDimDirectionalBC, Neumann0BC, NeumannBC, PeriodicBC,
+                  RobinBC, compose, decompose)
+from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
+from .fornberg import calculate_weights
+from .operators import CenteredDifference, DerivativeOperator, UpwindDifference
+
+mul = mul_
+
+__all__ = [
+    "CenteredDifference", "UpwindDifference", "DerivativeOperator", "calculate_weights",
+    "RobinBC", "DirichletBC", "Dirichlet0BC", "NeumannBC", "Neumann0BC", "MultiDimBC",
+    "MultiDimDirectionalBC", "ComposedMultiDimBC", "AffineBC", "PeriodicBC", "GeneralBC",
+    "compose", "decompose", "GhostDerivativeOperator", "DiffEqOperatorCombination",
+    "mul_", "mul", "plans_for", "Plan", "Batch", "RK4", "julia_size",
+]
+
+
+def library_path() -> str:
+    return _abi.LIB_PATH

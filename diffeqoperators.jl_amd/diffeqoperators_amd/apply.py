@@ -1,0 +1,322 @@
+"""Lowering of operators/BCs to libdop plans and their application on device tensors.
+
+Replaces the reference's evaluation entry points (derivative_operator_functions.jl:18-69,
+150-163, 203-802; ghost_derivative_operator.jl:15-37; composite_operators.jl:64-84):
+every call becomes one fused dop_apply on the current torch stream.  Tensors must be
+contiguous device tensors (float64, float32 or bfloat16); there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _abi
+from .bcs import AffineBC, ComposedMultiDimBC, MultiDimDirectionalBC
+from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
+from .operators import DerivativeOperator
+
+_STORAGE = {torch.float64: _abi.F64, torch.float32: _abi.F32, torch.bfloat16: _abi.BF16}
+
+
+def storage_of(t: torch.Tensor) -> int:
+    try:
+        return _STORAGE[t.dtype]
+    except KeyError:
+        raise TypeError(f"unsupported dtype {t.dtype}: use float64, float32 or bfloat16") from None
+
+
+def _check_device_tensor(t, name):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor on the GPU (no CPU path)")
+    if t.device.type != "cuda":
+        raise TypeError(f"{name} must live on the GPU (got {t.device}); this library has no CPU path")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def julia_size(t: torch.Tensor):
+    """Column-major size of a tensor (Julia's size(u))."""
+    return tuple(int(v) for v in reversed(t.shape))
+
+
+class Plan:
+    """An immutable libdop plan (dop_plan_create)."""
+
+    def __init__(self, desc: _abi.DopDesc, keep):
+        self._keep = keep
+        h = C.c_void_p()
+        _abi.check(_abi.lib().dop_plan_create(C.byref(desc), C.byref(h)), "dop_plan_create")
+        self.handle = h
+        self.storage = desc.storage
+        self.in_elems = _abi.lib().dop_plan_input_elems(h)
+        self.out_elems = _abi.lib().dop_plan_output_elems(h)
+        self.desc = desc
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _abi.lib().dop_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def apply(self, du: torch.Tensor, u: torch.Tensor, accumulate: bool = False, stream=None):
+        if du.numel() != self.out_elems or u.numel() != self.in_elems:
+            raise AssertionError("array sizes do not match the plan")
+        if storage_of(du) != self.storage or storage_of(u) != self.storage:
+            raise TypeError("tensor dtype does not match the plan's storage type")
+        if du.data_ptr() == u.data_ptr():
+            raise ValueError("du must not alias u")
+        if stream is None:
+            stream = torch.cuda.current_stream(u.device).cuda_stream
+        _abi.check(_abi.lib().dop_apply(self.handle, C.c_void_p(du.data_ptr()),
+                                        C.c_void_p(u.data_ptr()), int(bool(accumulate)),
+                                        C.c_void_p(stream)), "dop_apply")
+
+    def autotune(self, stream=None):
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        choice = C.c_int()
+        _abi.check(_abi.lib().dop_plan_autotune(self.handle, C.c_void_p(stream), C.byref(choice)),
+                   "dop_plan_autotune")
+        return choice.value
+
+
+def make_desc(terms, jsize, storage, *, input_kind, semantics, bcs=None, padded=(0, 0, 0),
+              transverse_view=0, accum=_abi.ACC_AUTO):
+    """terms: list of DerivativeOperator; bcs: {axis0: AffineBC}."""
+    ndim = len(jsize)
+    if ndim < 1 or ndim > 3:
+        raise ValueError("arrays of 1 to 3 dimensions are supported")
+    ops = (_abi.DopOperator * len(terms))()
+    keep = [ops]
+    for t, L in enumerate(terms):
+        if not L.N <= ndim:
+            raise AssertionError(f"operator axis {L.N} > ndims {ndim}")
+        ops[t] = L.c_struct(L.N - 1)
+        keep.append(L)
+    d = _abi.DopDesc()
+    d.ndim = ndim
+    d.storage = storage
+    d.accum = accum
+    d.input = input_kind
+    d.semantics = semantics
+    d.n_ops = len(terms)
+    for k in range(3):
+        d.size[k] = jsize[k] if k < ndim else 1
+        d.padded[k] = int(padded[k]) if k < ndim else 0
+    d.transverse_view = int(transverse_view)
+    d.ops = ops
+    for ax, bc in (bcs or {}).items():
+        q = d.bc[ax]
+        if len(bc.a_l) > _abi.MAX_BC_TAPS or len(bc.a_r) > _abi.MAX_BC_TAPS:
+            raise ValueError("boundary condition order too high")
+        q.present = 1
+        q.n_l, q.n_r = len(bc.a_l), len(bc.a_r)
+        for k, v in enumerate(bc.a_l):
+            q.a_l[k] = float(v)
+        for k, v in enumerate(bc.a_r):
+            q.a_r[k] = float(v)
+        q.b_l, q.b_r = bc.b_l, bc.b_r
+    return d, keep
+
+
+# ------------------------------------------------------------------------------------------
+# Q*u: which BC pads which axis, and which reference kernels apply
+
+def _bcs_of(Q, ndim, jsize):
+    """{axis0: AffineBC} of Q applied to an array of Julia size jsize, and the semantics."""
+    if isinstance(Q, AffineBC):
+        # 1-D: Q*u is a BoundaryPaddedVector (bc_operators.jl:188-191);
+        # N-D: BC*u = MultiDimBC{1}(BC, size(u))*u pads axis 1 (multi_dim_bc_operators.jl:210)
+        return {0: Q}, (_abi.SEM_BPV if ndim == 1 else _abi.SEM_LINES)
+    if ndim == 1:
+        raise AssertionError("multi-dimensional BCs need arrays of 2 or more dimensions")
+    if isinstance(Q, MultiDimDirectionalBC):
+        if tuple(Q.s) != tuple(jsize):
+            raise AssertionError("Size of the BCs array in the MultiDimBC is incorrect")
+        return {Q.D - 1: Q.bc}, _abi.SEM_LINES
+    if isinstance(Q, ComposedMultiDimBC):
+        out = {}
+        for b in Q.bcs:
+            if tuple(b.s) != tuple(jsize):
+                raise AssertionError(f"Size of the BCs array for dimension {b.D} is incorrect")
+            out[b.D - 1] = b.bc
+        return out, _abi.SEM_LINES
+    raise TypeError(f"not a boundary condition: {type(Q)}")
+
+
+def _ghost_groups(terms, ndim, jsize):
+    """Split (L, Q) terms into groups whose per-axis BCs agree (one plan per group)."""
+    groups = []   # [ops, bcs{axis: bc}, keys{axis: key}, sem]
+    for L, Q in terms:
+        bcs, sem = _bcs_of(Q, ndim, jsize)
+        ax = L.N - 1
+        if ax not in bcs:
+            raise AssertionError("the operator's axis is not padded by its boundary condition")
+        key = bcs[ax].key()
+        for g in groups:
+            if g[3] == sem and g[2].get(ax, key) == key:
+                g[0].append(L)
+                g[1][ax] = bcs[ax]
+                g[2][ax] = key
+                break
+        else:
+            groups.append([[L], {ax: bcs[ax]}, {ax: key}, sem])
+    return groups
+
+
+def _plan_cache(obj):
+    c = getattr(obj, "_plans", None)
+    if c is None:
+        c = {}
+        try:
+            obj._plans = c
+        except AttributeError:
+            pass
+    return c
+
+
+def ghost_plans(owner, terms, u: torch.Tensor, accum=_abi.ACC_AUTO):
+    """Plans evaluating sum_t L_t*Q_t*u (cached on owner)."""
+    jsize = julia_size(u)
+    storage = storage_of(u)
+    key = ("ghost", jsize, storage, accum)
+    cache = _plan_cache(owner)
+    if key not in cache:
+        with torch.cuda.device(u.device):
+            plans = []
+            for ops, bcs, _, sem in _ghost_groups(terms, len(jsize), jsize):
+                d, keep = make_desc(ops, jsize, storage, input_kind=_abi.INPUT_GHOST,
+                                    semantics=sem, bcs=bcs, accum=accum)
+                plans.append(Plan(d, keep + [bcs]))
+        cache[key] = plans
+    return cache[key]
+
+
+def _ghost_terms(A):
+    if isinstance(A, GhostDerivativeOperator):
+        return [(A.L, A.Q)]
+    if isinstance(A, DiffEqOperatorCombination) and all(
+            isinstance(op, GhostDerivativeOperator) for op in A.ops):
+        return [(op.L, op.Q) for op in A.ops]
+    return None
+
+
+def _apply_plans(plans, du, u, accumulate):
+    for i, p in enumerate(plans):
+        p.apply(du, u, accumulate=(accumulate or i > 0))
+
+
+def ghost_times(G, u):
+    _check_device_tensor(u, "u")
+    out = torch.empty_like(u)
+    _apply_plans(ghost_plans(G, _ghost_terms(G), u), out, u, False)
+    return out
+
+
+# ------------------------------------------------------------------------------------------
+# operators on padded arrays
+
+def _padded_plan(owner, ops, du, M, transverse_view=0, accum=_abi.ACC_AUTO):
+    osz, msz = julia_size(du), julia_size(M)
+    if len(osz) != len(msz):
+        raise AssertionError("du and M must have the same number of dimensions")
+    padded = []
+    for k in range(len(osz)):
+        if msz[k] == osz[k]:
+            padded.append(0)
+        elif msz[k] == osz[k] + 2:
+            padded.append(1)
+        else:
+            raise AssertionError("M must equal du in size or be padded by 2 along each axis")
+    for L in ops:
+        if not L.N <= len(osz) or not padded[L.N - 1]:
+            raise AssertionError("differentiated dimension must be padded")
+    storage = storage_of(M)
+    key = ("padded", tuple(id(L) for L in ops), osz, msz, storage, transverse_view, accum)
+    cache = _plan_cache(owner)
+    if key not in cache:
+        with torch.cuda.device(M.device):
+            d, keep = make_desc(list(ops), osz, storage, input_kind=_abi.INPUT_PADDED,
+                                semantics=_abi.SEM_LINES, padded=padded + [0] * (3 - len(padded)),
+                                transverse_view=transverse_view, accum=accum)
+            cache[key] = Plan(d, keep)
+    return cache[key]
+
+
+def derivative_times_array(L: DerivativeOperator, M):
+    """L * M (derivative_operator_functions.jl:150-163): M is padded along L's axis."""
+    _check_device_tensor(M, "M")
+    shape = list(M.shape)
+    shape[len(shape) - L.N] -= 2
+    out = torch.empty(shape, dtype=M.dtype, device=M.device)
+    _padded_plan(L, [L], out, M).apply(out, M)
+    return out
+
+
+def combination_times(A: DiffEqOperatorCombination, x):
+    """(L1 + L2 + ...) * x = sum(op * x) (composite_operators.jl:64-65)."""
+    _check_device_tensor(x, "x")
+    terms = _ghost_terms(A)
+    if terms is not None:
+        out = torch.empty_like(x)
+        _apply_plans(ghost_plans(A, terms, x), out, x, False)
+        return out
+    if all(isinstance(op, DerivativeOperator) for op in A.ops):
+        shapes = set()
+        for L in A.ops:
+            s = list(x.shape)
+            s[len(s) - L.N] -= 2
+            shapes.add(tuple(s))
+        if len(shapes) != 1:
+            raise ValueError("DimensionMismatch: the operators' results have different sizes")
+        out = torch.empty(shapes.pop(), dtype=x.dtype, device=x.device)
+        _padded_plan(A, A.ops, out, x).apply(out, x)
+        return out
+    parts = [op * x for op in A.ops]
+    out = parts[0]
+    for p in parts[1:]:
+        out = out + p
+    return out
+
+
+def mul_(du, A, u, overwrite=True, accum=_abi.ACC_AUTO):
+    """mul!(du, A, u): writes A*u into du (adds into du when overwrite=False)."""
+    _check_device_tensor(du, "du")
+    _check_device_tensor(u, "u")
+    accumulate = not overwrite
+    terms = _ghost_terms(A)
+    if terms is not None:
+        if tuple(du.shape) != tuple(u.shape):
+            raise AssertionError("du must have the unpadded size of u")
+        _apply_plans(ghost_plans(A, terms, u, accum), du, u, accumulate)
+        return du
+    if isinstance(A, DerivativeOperator):
+        # derivative_operator_functions.jl:18-69
+        _padded_plan(A, [A], du, u, accum=accum).apply(du, u, accumulate=accumulate)
+        return du
+    if isinstance(A, DiffEqOperatorCombination) and all(
+            isinstance(op, DerivativeOperator) for op in A.ops):
+        nd = du.dim()
+        if nd == 1:
+            tv = 0   # composite_operators.jl:76-84: plain per-op mul! and sum
+        elif nd == 2:
+            axes = {op.N for op in A.ops}
+            tv = 1 if len(axes) > 1 else 0   # derivative_operator_functions.jl:414-461
+        else:
+            tv = 1                           # derivative_operator_functions.jl:768-801
+        _padded_plan(A, A.ops, du, u, transverse_view=tv, accum=accum).apply(
+            du, u, accumulate=accumulate)
+        return du
+    raise TypeError(f"mul_ does not support {type(A)}")
+
+
+def plans_for(A, u, accum=_abi.ACC_AUTO):
+    """The libdop plans that `A * u` runs (for benchmarking and batching)."""
+    terms = _ghost_terms(A)
+    if terms is None:
+        raise TypeError("plans_for needs L*Q or a sum of L*Q")
+    return ghost_plans(A, terms, u, accum)

@@ -1,0 +1,205 @@
+"""Boundary conditions of the accelerated path: the affine (Robin-family) BCs of
+src/derivative_operators/bc_operators.jl and their N-D extensions from
+multi_dim_bc_operators.jl.
+
+An affine BC pads a line u with ghosts l = a_l . u[1:k] + b_l and r = a_r . u[end-k+1:end]
++ b_r (bc_operators.jl:188-191).  Dirichlet is Robin with beta = 0 evaluated through IEEE
+division by zero (a = +-0.0), exactly as in the reference (:176-178).
+"""
+from __future__ import annotations
+
+import numbers
+
+import numpy as np
+
+from .fornberg import calculate_weights
+
+
+class AffineBC:
+    """RobinBC / DirichletBC / NeumannBC (all Robin): a_l, b_l, a_r, b_r."""
+
+    def __init__(self, a_l, b_l, a_r, b_r):
+        self.a_l = np.asarray(a_l, dtype=np.float64)
+        self.b_l = float(b_l)
+        self.a_r = np.asarray(a_r, dtype=np.float64)
+        self.b_r = float(b_r)
+
+    def key(self):
+        return (tuple(self.a_l.tolist()), self.b_l, tuple(self.a_r.tolist()), self.b_r)
+
+    def __repr__(self):
+        return f"AffineBC(a_l={self.a_l.tolist()}, b_l={self.b_l}, a_r={self.a_r.tolist()}, b_r={self.b_r})"
+
+
+def _robin_atomic(l, r, dx, order=1):
+    al, bl, gl = (float(v) for v in l)
+    ar, br, gr = (float(v) for v in r)
+    order = int(order)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if np.ndim(dx) == 0:
+            # bc_operators.jl:26-39
+            dx = float(dx)
+            s = calculate_weights(1, 1.0, [float(k) for k in range(1, order + 2)])
+            dl = _div(al * dx, bl) + s[0]
+            dr = _div(ar * dx, br) - s[0]
+            a_l = [_div(-s[k], dl) for k in range(1, order + 1)]
+            a_r = [_div(s[k], dr) for k in range(order, 0, -1)]
+            b_l = _div(gl, al + _div(bl * s[0], dx))
+            b_r = _div(gr, ar - _div(br * s[0], dx))
+        else:
+            # bc_operators.jl:40-60
+            dxv = np.asarray(dx, dtype=np.float64)
+            ns = order + 1
+            dx_l, dx_r = dxv[:ns], dxv[len(dxv) - ns:]
+            s = calculate_weights(1, 1.0, [float(k) for k in range(1, ns + 1)])
+            denom_l = al + _div(bl * s[0], float(dx_l[0]))
+            denom_r = ar - _div(br * s[0], float(dx_r[ns - 1]))
+            a_l = [_div(-bl * s[k], denom_l * float(dx_l[k])) for k in range(1, ns)]
+            a_r = [_div(br * s[ns - 1 - m], denom_r * float(dx_r[m])) for m in range(order)]
+            b_l = _div(gl, denom_l)
+            b_r = _div(gr, denom_r)
+    return AffineBC(a_l, b_l, a_r, b_r)
+
+
+def _div(a, b):
+    """IEEE a / b (Python raises on float division by zero; Julia returns +-Inf / NaN)."""
+    return float(np.float64(a) / np.float64(b))
+
+
+class MultiDimDirectionalBC:
+    """An atomic BC extended over every line along axis D of an array of size s
+    (multi_dim_bc_operators.jl:54-57, 97-101)."""
+
+    def __init__(self, bc: AffineBC, D: int, s):
+        self.bc, self.D, self.s = bc, int(D), tuple(int(v) for v in s)
+
+    def __add__(self, other):
+        return compose(self, other)
+
+
+class ComposedMultiDimBC:
+    """One directional BC per axis (multi_dim_bc_operators.jl:59-61)."""
+
+    def __init__(self, bcs):
+        self.bcs = tuple(bcs)   # sorted by axis
+
+    @property
+    def s(self):
+        return self.bcs[0].s
+
+
+def compose(*BCs):
+    """multi_dim_bc_operators.jl:178-192."""
+    N = len(BCs[0].s)
+    Ds = [b.D for b in BCs]
+    if len(BCs) != N:
+        raise ValueError("ArgumentError: There must be enough BCs to cover every dimension - "
+                         "check that the number of MultiDimBCs == N")
+    if len(set(Ds)) != len(Ds):
+        raise ValueError("ArgumentError: There are multiple boundary conditions that extend "
+                         "along the same dimension - make sure every dimension has a unique extension")
+    return ComposedMultiDimBC(sorted(BCs, key=lambda b: b.D))
+
+
+def decompose(Q: ComposedMultiDimBC):
+    return Q.bcs
+
+
+def is_bc(x) -> bool:
+    return isinstance(x, (AffineBC, MultiDimDirectionalBC, ComposedMultiDimBC))
+
+
+def _is_size(s):
+    return isinstance(s, (tuple, list)) and all(isinstance(v, numbers.Integral) for v in s)
+
+
+class _DimBC:
+    """`Name(...)` and `Name[dim](...)` like Julia's `Name` / `Name{dim}`."""
+
+    def __init__(self, fn, fn_dim):
+        self._fn, self._fn_dim = fn, fn_dim
+
+    def __call__(self, *a, **k):
+        return self._fn(*a, **k)
+
+    def __getitem__(self, dim):
+        return lambda *a, **k: self._fn_dim(int(dim), *a, **k)
+
+
+def _robin(l, r, dx, order=1, s=None):
+    if s is None:
+        return _robin_atomic(l, r, dx, order)
+    # RobinBC(l, r, dxyz, order, s): one directional BC per axis (:141-149)
+    return tuple(MultiDimDirectionalBC(_robin_atomic(l, r, dx[dim], order), dim + 1, s)
+                 for dim in range(len(s)))
+
+
+def _robin_dim(dim, l, r, dx, order, s):
+    return MultiDimDirectionalBC(_robin_atomic(l, r, dx, order), dim, s)
+
+
+def _dirichlet(al, ar, s=None):
+    if s is None:   # bc_operators.jl:176-178
+        return _robin_atomic((1.0, 0.0, al), (1.0, 0.0, ar), 1.0, 2)
+    # multi_dim_bc_operators.jl:126-128: dxyz = fill(1, ndims)
+    return _robin((1.0, 0.0, al), (1.0, 0.0, ar), [1.0] * len(s), 2, s)
+
+
+def _dirichlet_dim(dim, al, ar, s):
+    return _robin_dim(dim, (1.0, 0.0, al), (1.0, 0.0, ar), 1.0, 2, s)
+
+
+def _dirichlet0(T=float, s=None):
+    return _dirichlet(0.0, 0.0, s)
+
+
+def _dirichlet0_dim(dim, T, s):
+    return _dirichlet_dim(dim, 0.0, 0.0, s)
+
+
+def _neumann(alpha, dx, order=1, s=None):
+    return _robin((0.0, 1.0, alpha[0]), (0.0, 1.0, alpha[1]), dx, order, s)
+
+
+def _neumann_dim(dim, alpha, dx, order, s):
+    return _robin_dim(dim, (0.0, 1.0, alpha[0]), (0.0, 1.0, alpha[1]), dx, order, s)
+
+
+def _neumann0(*args):
+    # Neumann0BC(dx, order=1)  |  Neumann0BC(T, dxyz, order, s)
+    if len(args) == 4:
+        _, dxyz, order, s = args
+        return _neumann((0.0, 0.0), dxyz, order, s)
+    dx = args[0]
+    order = args[1] if len(args) > 1 else 1
+    return _neumann((0.0, 0.0), dx, order)
+
+
+def _neumann0_dim(dim, T, dx, order, s):
+    return _neumann_dim(dim, (0.0, 0.0), dx, order, s)
+
+
+def _multidim(bc, s):
+    if not _is_size(s):
+        raise TypeError("MultiDimBC(bc, s) needs the size tuple s")
+    return tuple(MultiDimDirectionalBC(bc, dim + 1, s) for dim in range(len(s)))
+
+
+def _multidim_dim(dim, bc, s):
+    return MultiDimDirectionalBC(bc, dim, s)
+
+
+RobinBC = _DimBC(_robin, _robin_dim)
+DirichletBC = _DimBC(_dirichlet, _dirichlet_dim)
+Dirichlet0BC = _DimBC(_dirichlet0, _dirichlet0_dim)
+NeumannBC = _DimBC(_neumann, _neumann_dim)
+Neumann0BC = _DimBC(_neumann0, _neumann0_dim)
+MultiDimBC = _DimBC(_multidim, _multidim_dim)
+
+
+def PeriodicBC(*args, **kwargs):
+    raise NotImplementedError("PeriodicBC is not on the accelerated path of this library")
+
+
+def GeneralBC(*args, **kwargs):
+    raise NotImplementedError("GeneralBC is not on the accelerated path of this library")

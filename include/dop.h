@@ -1,0 +1,189 @@
+/*
+ * dop.h -- C ABI of libdop.so, the MI355X (gfx950) operator-application library.
+ *
+ * It replaces the application step of DiffEqOperators.jl's lazy finite-difference
+ * operators: the host (Julia via `ccall`, Python via ctypes) keeps building the
+ * operators and boundary conditions exactly as before and hands their FP64 tables to
+ * dop_plan_create(); dop_apply() then evaluates
+ *
+ *      du  = sum_t  L_t * Q * u          (GhostDerivativeOperator / Combination)
+ *      du  = sum_t  L_t * M              (DerivativeOperator on a padded array)
+ *
+ * on device memory in one fused pass.  Paths in citations are relative to the
+ * DiffEqOperators.jl v4.45.0 source tree.
+ *
+ * Conventions
+ *   - Arrays are column-major (Julia order): size[0] is the fastest axis.  An operator's
+ *     `axis` is Julia's type parameter N minus one.
+ *   - u / du are device pointers to contiguous arrays of the plan's storage type.  The
+ *     caller owns them and the stream.  du must not alias u.
+ *   - Every function returns DOP_OK (0) or a negative DOP_E* code; dop_last_error()
+ *     returns a thread-local message for the last failure.  No exception crosses the ABI.
+ *   - Plans are immutable after creation: concurrent dop_apply calls on one plan from
+ *     several threads/streams are safe.
+ */
+#ifndef DOP_H
+#define DOP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DOP_ABI_VERSION 1
+
+/* error codes */
+#define DOP_OK 0
+#define DOP_EINVAL (-1)       /* bad shape / argument: the reference's AssertionError,
+                                 derivative_operator_functions.jl:33-40 */
+#define DOP_EDIM (-2)         /* length mismatch: DimensionMismatch, :182-184 */
+#define DOP_EHIP (-3)         /* HIP runtime error */
+#define DOP_ENOMEM (-4)       /* device allocation failed */
+#define DOP_EUNSUPPORTED (-5) /* valid in the reference, not provided by this library */
+#define DOP_EBUDGET (-6)      /* request exceeds the device-memory budget */
+
+/* storage types of u / du */
+#define DOP_F64 0
+#define DOP_F32 1
+#define DOP_BF16 2
+
+/* accumulation */
+#define DOP_ACC_AUTO 0        /* f64 for f64 storage, f32 (+Kahan when d >= 3) otherwise */
+#define DOP_ACC_F64 1
+#define DOP_ACC_F32 2
+#define DOP_ACC_F32_KAHAN 3   /* f32 with compensated (TwoProduct/TwoSum) accumulation */
+
+/* input kinds */
+#define DOP_INPUT_GHOST 0     /* u unpadded; ghosts from the per-axis affine BCs (Q*u) */
+#define DOP_INPUT_PADDED 1    /* u is the padded array M of the plain mul! */
+
+/* which reference kernels define the row arithmetic */
+#define DOP_SEM_LINES 0       /* AbstractVector kernels, convolutions.jl:27-362 (N-D Q*u,
+                                 plain padded mul!) */
+#define DOP_SEM_BPV 1         /* BoundaryPaddedVector kernels, convolutions.jl:367-730
+                                 (1-D Q*u with an atomic BC) */
+
+/* coeff_func kind, for the non-uniform upwind interior branch (convolutions.jl:234-253) */
+#define DOP_COEFF_VECTOR 0    /* Vector / Function coeff_func: wind from each coefficient */
+#define DOP_COEFF_NUM_POS 1   /* coeff_func::Number >= 0 */
+#define DOP_COEFF_NUM_NEG 2   /* coeff_func::Number <  0 */
+
+/* One DerivativeOperator's fields (derivative_operator.jl:14-29).  Table layouts
+ * (row-major, FP64):
+ *   centered, uniform dx:      stencil_coefs[SL], low/high_boundary_coefs[BPC][BSL]
+ *   centered, vector dx:       stencil_coefs[len-2BPC][SL], low/high[BPC][BSL]
+ *   upwind, uniform dx:        stencil_coefs[SL], low[BPC][BSL], high[BPC+offside][BSL]
+ *   upwind, vector dx:         stencil_coefs[2][len-2BPC][SL], low[2][BPC][BSL],
+ *                              high[2][BPC+offside][BSL]  (first index: 0 upwind, 1 downwind)
+ */
+typedef struct dop_operator {
+    int32_t axis;                     /* N - 1 */
+    int32_t wind;                     /* Wind type parameter: 0 centered, 1 upwind */
+    int32_t derivative_order;
+    int32_t approximation_order;
+    int32_t len;
+    int32_t stencil_length;
+    int32_t boundary_stencil_length;
+    int32_t boundary_point_count;
+    int32_t offside;
+    int32_t uniform_grid;             /* dx isa Number */
+    int32_t coeff_kind;               /* DOP_COEFF_* */
+    int32_t reserved;
+    const double* stencil_coefs;
+    const double* low_boundary_coefs;
+    const double* high_boundary_coefs;
+    const double* coefficients;       /* [len] */
+} dop_operator;
+
+/* An affine BC along one axis (RobinBC and its Dirichlet/Neumann special cases,
+ * bc_operators.jl:21-61, 173-183): ghost_l = a_l . u[1:n_l] + b_l,
+ * ghost_r = a_r . u[end-n_r+1:end] + b_r, identical for every line of that axis
+ * (multi_dim_bc_operators.jl:141-149). */
+#define DOP_MAX_BC_TAPS 8
+typedef struct dop_affine_bc {
+    int32_t present;                  /* 0: axis has no BC (must carry no operator) */
+    int32_t n_l, n_r;
+    int32_t reserved;
+    double a_l[DOP_MAX_BC_TAPS];
+    double b_l;
+    double a_r[DOP_MAX_BC_TAPS];
+    double b_r;
+} dop_affine_bc;
+
+typedef struct dop_desc {
+    int32_t ndim;                     /* 1..3 */
+    int32_t storage;                  /* DOP_F64 / DOP_F32 / DOP_BF16 */
+    int32_t accum;                    /* DOP_ACC_* */
+    int32_t input;                    /* DOP_INPUT_* */
+    int32_t semantics;                /* DOP_SEM_* */
+    int32_t n_ops;
+    int64_t size[3];                  /* unpadded size of du (and of u for GHOST) */
+    /* PADDED input: padded[k] = 1 if M is size[k]+2 along axis k.  transverse_view = 1
+     * reproduces the views 1:n of the composite mul! (derivative_operator_functions.jl:
+     * 429-460, 780-800) instead of trimming padded transverse axes to 2:end-1 (:48-57). */
+    int32_t padded[3];
+    int32_t transverse_view;
+    const dop_operator* ops;          /* [n_ops], applied as sum_t */
+    dop_affine_bc bc[3];              /* GHOST input: per axis */
+} dop_desc;
+
+typedef struct dop_plan dop_plan;
+
+int dop_version(void);
+const char* dop_last_error(void);
+
+/* Builds the fused per-axis row programs from the operators' own tables and uploads them.
+ * Replaces the operator/BC side of `L*Q` (ghost_derivative_operator.jl:7-13),
+ * `L1 + L2` (composite_operators.jl:31-46) and `compose` (multi_dim_bc_operators.jl:
+ * 178-192): the device-side form of what those lazy objects hold. */
+int dop_plan_create(const dop_desc* desc, dop_plan** out);
+void dop_plan_destroy(dop_plan* plan);
+
+/* Number of bytes of u and du the plan reads/writes per dop_apply (storage elements). */
+int64_t dop_plan_input_elems(const dop_plan* plan);
+int64_t dop_plan_output_elems(const dop_plan* plan);
+
+/* du = sum_t L_t * Q * u  (accumulate = 0), or du += ... (accumulate = 1).
+ * Replaces: mul!(out, A::GhostDerivativeOperator, u) (ghost_derivative_operator.jl:15-24),
+ *           *(A::GhostDerivativeOperator, u) (:26-37),
+ *           *(L::DiffEqOperatorCombination, x) (composite_operators.jl:64-65),
+ *           mul!(du, L::DerivativeOperator, M; overwrite) (derivative_operator_functions.jl:
+ *             18-25, convolutions.jl:17-22), *(L, M) (:150-163),
+ *           composite mul! (derivative_operator_functions.jl:203, 466).
+ * accumulate = 0 never reads du (the `!overwrite * x_temp[i]` strong zero,
+ * convolutions.jl:49).  stream: hipStream_t (NULL = default stream). */
+int dop_apply(const dop_plan* plan, void* du, const void* u, int accumulate, void* stream);
+
+/* ---- batching (no counterpart in the reference: it applies one operator per call) ----
+ * A batch binds n independent plans (any mix of ndim, sizes, operators, BCs; one storage
+ * type) to their du / u device pointers once; dop_batch_apply then evaluates all of them
+ * in ONE persistent-kernel launch. */
+typedef struct dop_batch dop_batch;
+int dop_batch_create(int n, const dop_plan* const* plans, void* const* du,
+                     const void* const* u, dop_batch** out);
+int dop_batch_apply(const dop_batch* batch, int accumulate, void* stream);
+void dop_batch_destroy(dop_batch* batch);
+
+/* ---- explicit time stepping (no counterpart in the reference; its RHS is called from
+ * OrdinaryDiffEq, heat_equation.jl:30-33) ----
+ * A batch of method-of-lines problems du/dt = scale_i * (sum_t L_t Q u) advanced with
+ * classical RK4, every stage of the whole batch one persistent launch, the four stages of
+ * a step captured once in a hipGraph and replayed.  Storage f32 or f64. */
+typedef struct dop_rk4 dop_rk4;
+int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale,
+                   const double* dt, void* const* u, void* stream, dop_rk4** out);
+int dop_rk4_step(dop_rk4* rk, int nsteps, void* stream);   /* advances every problem */
+void dop_rk4_destroy(dop_rk4* rk);
+
+/* ---- tile autotuner (no counterpart in the reference) ----
+ * Times the candidate tile shapes of the plan's fused kernel on scratch buffers and keeps
+ * the fastest in the plan (and in a process-wide cache keyed by dtype, window and
+ * extents).  Returns the chosen candidate index via *choice (may be NULL). */
+int dop_plan_autotune(dop_plan* plan, void* stream, int* choice);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

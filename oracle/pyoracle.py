@@ -1,0 +1,268 @@
+"""ctypes binding of the FP64 CPU restatement (oracle/dop_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker or the timed CPU baseline.  The product package never
+imports this module.
+
+Arrays follow the reference's column-major order: a Julia array of size (n1, n2, n3)
+is a C-contiguous numpy array of shape (n3, n2, n1).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libdop_oracle.so")
+
+COEFF_VECTOR, COEFF_NUM_POS, COEFF_NUM_NEG = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+
+
+class OrcOp(C.Structure):
+    _fields_ = [
+        ("axis", C.c_int), ("wind", C.c_int), ("d", C.c_int), ("a", C.c_int),
+        ("len", C.c_int), ("SL", C.c_int), ("BSL", C.c_int), ("BPC", C.c_int),
+        ("offside", C.c_int), ("uniform", C.c_int), ("coeff_kind", C.c_int),
+        ("stencil", _dp), ("low", _dp), ("high", _dp), ("coeff", _dp),
+    ]
+
+
+class OrcBC(C.Structure):
+    _fields_ = [
+        ("nl", C.c_int), ("nr", C.c_int), ("a_l", _dp), ("b_l", C.c_double),
+        ("a_r", _dp), ("b_r", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {_LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(_LIB_PATH)
+        i, d = C.c_int, C.c_double
+        ip = C.POINTER(C.c_int)
+        llp = C.POINTER(C.c_longlong)
+        L.orc_calculate_weights.argtypes = [i, d, _dp, i, _dp]
+        L.orc_centered_sizes.argtypes = [i, i, ip, ip, ip]
+        L.orc_upwind_sizes.argtypes = [i, i, i, ip, ip, ip]
+        L.orc_centered_uniform.argtypes = [i, i, d, i, _dp, _dp, _dp]
+        L.orc_centered_nonuniform.argtypes = [i, i, _dp, i, _dp, _dp, _dp]
+        L.orc_upwind_uniform.argtypes = [i, i, d, i, i, _dp, _dp, _dp]
+        L.orc_upwind_nonuniform.argtypes = [i, i, _dp, i, i, _dp, _dp, _dp]
+        L.orc_robin_uniform.argtypes = [d, d, d, d, d, d, d, i, _dp, _dp, _dp, _dp]
+        L.orc_robin_nonuniform.argtypes = [d, d, d, d, d, d, _dp, i, i, _dp, _dp, _dp, _dp]
+        L.orc_mul_line.argtypes = [C.POINTER(OrcOp), _dp, _dp, i]
+        L.orc_mul_line_bpv.argtypes = [C.POINTER(OrcOp), d, d, _dp, _dp, i]
+        L.orc_mul_nd.argtypes = [C.POINTER(OrcOp), i, llp, _dp, llp, _dp, i]
+        L.orc_ghost_apply.argtypes = [i, C.POINTER(OrcOp), i, llp, C.POINTER(OrcBC), _dp, _dp, i]
+        L.orc_composite_mul_padded.argtypes = [i, C.POINTER(OrcOp), i, llp, _dp, _dp]
+        L.orc_set_threads.argtypes = [i]
+        L.orc_get_threads.restype = i
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ValueError(f"oracle {what} failed with code {rc}")
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def calculate_weights(order: int, x0: float, x) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty(len(x))
+    _check(lib().orc_calculate_weights(order, float(x0), _ptr(x), len(x), _ptr(out)), "weights")
+    return out
+
+
+class Op:
+    """An oracle DerivativeOperator: FP64 tables built by the C restatement."""
+
+    def __init__(self, wind, d, a, dx, n, axis=0, offside=0, coeff=1.0):
+        L = lib()
+        SL, BSL, BPC = C.c_int(), C.c_int(), C.c_int()
+        self.wind, self.d, self.a, self.len, self.axis = int(wind), d, a, n, axis
+        self.offside = offside
+        self.uniform = np.ndim(dx) == 0
+        if wind:
+            _check(L.orc_upwind_sizes(d, a, offside, C.byref(SL), C.byref(BSL), C.byref(BPC)), "sizes")
+        else:
+            _check(L.orc_centered_sizes(d, a, C.byref(SL), C.byref(BSL), C.byref(BPC)), "sizes")
+        self.SL, self.BSL, self.BPC = SL.value, BSL.value, BPC.value
+        nin = n - 2 * self.BPC
+        if wind:
+            ns = self.SL if self.uniform else 2 * nin * self.SL
+            nl = self.BPC * self.BSL * (1 if self.uniform else 2)
+            nh = (self.BPC + offside) * self.BSL * (1 if self.uniform else 2)
+        else:
+            ns = self.SL if self.uniform else nin * self.SL
+            nl = nh = self.BPC * self.BSL
+        self.stencil = np.zeros(max(ns, 1))
+        self.low = np.zeros(max(nl, 1))
+        self.high = np.zeros(max(nh, 1))
+        if self.uniform:
+            dx = float(dx)
+            f = L.orc_upwind_uniform if wind else L.orc_centered_uniform
+            args = (d, a, dx, n, offside) if wind else (d, a, dx, n)
+        else:
+            self.dxv = np.ascontiguousarray(dx, dtype=np.float64)
+            if len(self.dxv) != n + 1:
+                raise ValueError("non-uniform dx needs len+1 spacings")
+            f = L.orc_upwind_nonuniform if wind else L.orc_centered_nonuniform
+            args = (d, a, _ptr(self.dxv), n, offside) if wind else (d, a, _ptr(self.dxv), n)
+        _check(f(*args, _ptr(self.stencil), _ptr(self.low), _ptr(self.high)), "constructor")
+        if np.ndim(coeff) == 0:
+            self.coeff = np.zeros(n) + float(coeff)
+            self.coeff_kind = COEFF_NUM_POS if float(coeff) >= 0 else COEFF_NUM_NEG
+        else:
+            self.coeff = np.zeros(n) + np.asarray(coeff, dtype=np.float64)
+            self.coeff_kind = COEFF_VECTOR
+
+    def scaled(self, c):
+        """c * A (derivative_operator_functions.jl:165-197): scales the coefficients."""
+        import copy
+        B = copy.copy(self)
+        B.coeff = (np.asarray(c, dtype=np.float64) * self.coeff).astype(np.float64)
+        return B
+
+    def struct(self) -> OrcOp:
+        return OrcOp(self.axis, self.wind, self.d, self.a, self.len, self.SL, self.BSL,
+                     self.BPC, self.offside, int(self.uniform), self.coeff_kind,
+                     _ptr(self.stencil), _ptr(self.low), _ptr(self.high), _ptr(self.coeff))
+
+    # 1-D application
+    def mul_padded(self, x, out=None, overwrite=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.len) if out is None else out
+        s = self.struct()
+        _check(lib().orc_mul_line(C.byref(s), _ptr(x), _ptr(out), int(overwrite)), "mul_line")
+        return out
+
+    def mul_bpv(self, l, r, u, out=None, overwrite=True):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(self.len) if out is None else out
+        s = self.struct()
+        _check(lib().orc_mul_line_bpv(C.byref(s), float(l), float(r), _ptr(u), _ptr(out),
+                                      int(overwrite)), "mul_bpv")
+        return out
+
+    def matrix(self, bpv=False):
+        """len x (len+2) matrix by multiplying unit vectors (the reference tests'
+        convert_by_multiplication, derivative_operators_interface.jl:102-117)."""
+        n = self.len
+        A = np.zeros((n, n + 2))
+        for j in range(n + 2):
+            v = np.zeros(n + 2)
+            v[j] = 1.0
+            if bpv:
+                A[:, j] = self.mul_bpv(v[0], v[-1], v[1:-1])
+            else:
+                A[:, j] = self.mul_padded(v)
+        return A
+
+
+class BC:
+    """An affine (Robin-family) BC: ghost = a . u[1:k] + b on each side."""
+
+    def __init__(self, a_l, b_l, a_r, b_r):
+        self.a_l = np.ascontiguousarray(a_l, dtype=np.float64)
+        self.a_r = np.ascontiguousarray(a_r, dtype=np.float64)
+        self.b_l, self.b_r = float(b_l), float(b_r)
+
+    def struct(self) -> OrcBC:
+        return OrcBC(len(self.a_l), len(self.a_r), _ptr(self.a_l), self.b_l, _ptr(self.a_r), self.b_r)
+
+    def pad(self, u):
+        u = np.asarray(u, dtype=np.float64)
+        gl = float(np.dot(self.a_l, u[: len(self.a_l)]) + self.b_l)
+        gr = float(np.dot(self.a_r, u[len(u) - len(self.a_r):]) + self.b_r)
+        return gl, gr
+
+
+def robin(l, r, dx, order=1) -> BC:
+    """RobinBC(l, r, dx, order): bc_operators.jl:26-60."""
+    al, bl, cl = map(float, l)
+    ar, br, cr = map(float, r)
+    a_l = np.zeros(order)
+    a_r = np.zeros(order)
+    b_l = C.c_double()
+    b_r = C.c_double()
+    if np.ndim(dx) == 0:
+        _check(lib().orc_robin_uniform(al, bl, cl, ar, br, cr, float(dx), order, _ptr(a_l),
+                                       C.byref(b_l), _ptr(a_r), C.byref(b_r)), "robin")
+    else:
+        dxv = np.ascontiguousarray(dx, dtype=np.float64)
+        _check(lib().orc_robin_nonuniform(al, bl, cl, ar, br, cr, _ptr(dxv), len(dxv), order,
+                                          _ptr(a_l), C.byref(b_l), _ptr(a_r), C.byref(b_r)), "robin")
+    return BC(a_l, b_l.value, a_r, b_r.value)
+
+
+def dirichlet(vl, vr) -> BC:
+    """DirichletBC(αl, αr) = RobinBC((1,0,αl),(1,0,αr), 1.0, 2): bc_operators.jl:176-178."""
+    return robin((1.0, 0.0, vl), (1.0, 0.0, vr), 1.0, 2)
+
+
+def dirichlet0() -> BC:
+    return dirichlet(0.0, 0.0)
+
+
+def neumann(g, dx, order=1) -> BC:
+    """NeumannBC((γl, γr), dx, order) = RobinBC((0,1,γl),(0,1,γr), dx, order): :173-175."""
+    return robin((0.0, 1.0, g[0]), (0.0, 1.0, g[1]), dx, order)
+
+
+def _ll(a):
+    arr = (C.c_longlong * len(a))(*[int(v) for v in a])
+    return arr
+
+
+def ghost_apply(ops, bcs, u, bpv_1d=True) -> np.ndarray:
+    """sum_t (L_t * Q) * u with Q the composed per-axis BCs; u column-major (numpy shape
+    reversed from the Julia size)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    size = list(u.shape[::-1])
+    ndim = len(size)
+    out = np.zeros_like(u)
+    structs = (OrcOp * len(ops))(*[o.struct() for o in ops])
+    bstructs = (OrcBC * ndim)(*[b.struct() for b in bcs])
+    _check(lib().orc_ghost_apply(len(ops), structs, ndim, _ll(size), bstructs, _ptr(u),
+                                 _ptr(out), int(bpv_1d)), "ghost_apply")
+    return out
+
+
+def mul_nd(op, M, out_shape, out=None, overwrite=True) -> np.ndarray:
+    """_mul!(out, L, M): out_shape / M in numpy (reversed) order."""
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    out = np.zeros(out_shape) if out is None else out
+    s = op.struct()
+    osz = list(out.shape[::-1])
+    msz = list(M.shape[::-1])
+    _check(lib().orc_mul_nd(C.byref(s), len(osz), _ll(osz), _ptr(out), _ll(msz), _ptr(M),
+                            int(overwrite)), "mul_nd")
+    return out
+
+
+def composite_mul_padded(ops, M) -> np.ndarray:
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    out_shape = tuple(s - 2 for s in M.shape)
+    out = np.zeros(out_shape)
+    structs = (OrcOp * len(ops))(*[o.struct() for o in ops])
+    osz = list(out_shape[::-1])
+    _check(lib().orc_composite_mul_padded(len(ops), structs, len(osz), _ll(osz), _ptr(M),
+                                          _ptr(out)), "composite")
+    return out

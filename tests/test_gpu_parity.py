@@ -1,0 +1,228 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the same seeded
+inputs.  Tolerances (normwise, max|got - ref| / max|ref|): 1e-12 for FP64 storage; for
+FP32 / BF16 storage the oracle runs on the same quantized input and the bounds are 1e-5
+(FP32) and 1e-2 (BF16, whose output rounding alone is 2^-9)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+import diffeqoperators_amd as D
+from diffeqoperators_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.float64: 1e-12, torch.float32: 1e-5, torch.bfloat16: 1e-2}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_threads():
+    po.set_threads(min(16, os.cpu_count() or 1))
+    yield
+    po.set_threads(1)
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def oop(L):
+    o = po.Op(int(L.wind), L.derivative_order, L.approximation_order, L.dx, L.len,
+              axis=L.N - 1, offside=L.offside)
+    o.coeff = L.coefficients.copy()
+    o.coeff_kind = L.coeff_kind
+    return o
+
+
+def obc(bc):
+    return po.BC(bc.a_l, bc.b_l, bc.a_r, bc.b_r)
+
+
+def to_dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda").to(dtype)
+
+
+def host64(t):
+    return t.to(torch.float64).cpu().numpy()
+
+
+# ---------------------------------------------------------------------------------------
+# workload 1: 1-D heat-equation RHS (heat_equation.jl:17-30)
+
+def test_heat_equation_rhs_n100():
+    n = 100
+    h = 1.0 / (n + 1)
+    x = h * np.arange(1, n + 1)
+    rng = np.random.default_rng(0)
+    u = np.sin(2 * np.pi * x) + rng.normal(0, 1e-3, n)
+    Δ = D.CenteredDifference(2, 2, h, n)
+    bc = D.Dirichlet0BC(float)
+    print("case: heat N=100 f64", flush=True)
+    got = host64(Δ * bc * to_dev(u))
+    ref = po.ghost_apply([oop(Δ)], [obc(bc)], u)
+    assert rel(got, ref) < 1e-12
+    # mul! form into an existing buffer
+    du = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+    D.mul_(du, Δ * bc, to_dev(u))
+    assert rel(host64(du), ref) < 1e-12
+
+
+@pytest.mark.parametrize("d,a", [(1, 2), (2, 2), (2, 4), (3, 2), (4, 4), (2, 6), (4, 8), (3, 8)])
+@pytest.mark.parametrize("coeff", ["const", "vector"])
+def test_1d_centered(d, a, coeff):
+    rng = np.random.default_rng(d * 10 + a)
+    n = 257
+    u = rng.standard_normal(n)
+    c = 1.0 if coeff == "const" else rng.standard_normal(n)
+    L = D.CenteredDifference(d, a, 0.01, n, c)
+    bc = D.RobinBC((1.0, 0.5, 0.2), (0.8, 0.3, -0.4), 0.01, 2)
+    print(f"case: 1d centered d={d} a={a} {coeff}", flush=True)
+    got = host64((L * bc) * to_dev(u))
+    ref = po.ghost_apply([oop(L)], [obc(bc)], u, bpv_1d=True)
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("d,a,off", [(1, 1, 0), (1, 2, 0), (1, 3, 0), (2, 2, 0), (2, 3, 1),
+                                     (1, 4, 2), (3, 2, 0)])
+@pytest.mark.parametrize("grid", ["uniform", "irregular"])
+@pytest.mark.parametrize("wind", ["pos", "neg", "mixed", "scaled"])
+def test_1d_upwind(d, a, off, grid, wind):
+    rng = np.random.default_rng(d * 100 + a * 10 + off)
+    n = 301
+    dx = 0.01 if grid == "uniform" else 0.005 * (1 + 0.5 * rng.random(n + 1))
+    c = {"pos": 1.3, "neg": -0.7, "mixed": rng.standard_normal(n), "scaled": 1.0}[wind]
+    L = D.UpwindDifference(d, a, dx, n, c, offside=off)
+    if wind == "scaled":
+        L = -2.0 * L
+    bc = D.NeumannBC((0.3, -0.2), dx, 1)
+    u = rng.standard_normal(n)
+    print(f"case: 1d upwind d={d} a={a} off={off} {grid} {wind}", flush=True)
+    got = host64((L * bc) * to_dev(u))
+    ref = po.ghost_apply([oop(L)], [obc(bc)], u, bpv_1d=True)
+    assert rel(got, ref) < 1e-12
+
+
+# ---------------------------------------------------------------------------------------
+# workload 2: 2-D uniform Laplacian, Robin(1, 0.5, 0) on every edge, FP64
+
+def laplacian_robin(n, h):
+    Lx = D.CenteredDifference[1](2, 4, h, n)
+    Ly = D.CenteredDifference[2](2, 4, h, n)
+    Q = D.compose(*D.RobinBC((1.0, 0.5, 0.0), (1.0, 0.5, 0.0), (h, h), 1, (n, n)))
+    return (Lx + Ly) * Q, [Lx, Ly], Q
+
+
+@pytest.mark.parametrize("n", [64, 1000, 4096])
+def test_2d_laplacian_robin_f64(n):
+    h = 1.0 / (n + 1)
+    A, ops, Q = laplacian_robin(n, h)
+    rng = np.random.default_rng(1)
+    u = rng.uniform(-1, 1, (n, n))
+    print(f"case: 2d laplacian robin n={n} f64", flush=True)
+    got = host64(A * to_dev(u))
+    ref = po.ghost_apply([oop(L) for L in ops], [obc(b.bc) for b in Q.bcs], u)
+    assert rel(got, ref) < 1e-12
+
+
+def test_2d_laplacian_linearity_full_size():
+    # a size-independent property at the full 4096^2 size: the operator (gamma = 0) is linear
+    n = 4096
+    A, _, _ = laplacian_robin(n, 1.0 / (n + 1))
+    g = torch.Generator(device="cuda").manual_seed(7)
+    u = torch.rand((n, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    v = torch.rand((n, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    lhs = A * (0.75 * u - 1.5 * v)
+    rhs = 0.75 * (A * u) - 1.5 * (A * v)
+    err = ((lhs - rhs).abs().max() / rhs.abs().max()).item()
+    assert err < 1e-13
+
+
+# ---------------------------------------------------------------------------------------
+# workload 3: 3-D non-uniform CenteredDifference(2,6) + UpwindDifference(1,3) per axis
+
+def nonuniform_3d(n, seed=10):
+    h = 1.0 / (n + 1)
+    ops = []
+    for k in (1, 2, 3):
+        r = np.random.default_rng(seed + k)
+        dx = h * (1 + 0.5 * r.random(n + 1))
+        v = np.random.default_rng(20 + k).standard_normal(n)
+        ops += [D.CenteredDifference[k](2, 6, dx, n), D.UpwindDifference[k](1, 3, dx, n, -v)]
+    Q = D.compose(*D.Dirichlet0BC(float, (n, n, n)))
+    A = ops[0]
+    for L in ops[1:]:
+        A = A + L
+    return A * Q, ops, Q
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("n", [48, 160])
+def test_3d_nonuniform(dtype, n):
+    A, ops, Q = nonuniform_3d(n)
+    rng = np.random.default_rng(3)
+    u64 = rng.uniform(-1, 1, (n, n, n))
+    ud = to_dev(u64, dtype)
+    uq = host64(ud)                       # quantized input, exactly representable in f64
+    print(f"case: 3d nonuniform n={n} {dtype}", flush=True)
+    got = host64(A * ud)
+    ref = po.ghost_apply([oop(L) for L in ops], [obc(b.bc) for b in Q.bcs], uq)
+    err = rel(got, ref)
+    print(f"  max rel err vs FP64 oracle: {err:.3e}", flush=True)
+    assert err < TOL[dtype]
+
+
+# ---------------------------------------------------------------------------------------
+# padded arrays: L*M, mul!(du, L, M; overwrite), composite mul!
+
+def test_padded_single_operator_and_accumulate():
+    rng = np.random.default_rng(5)
+    n1, n2 = 300, 77
+    L = D.CenteredDifference[1](4, 4, 0.1, n1, rng.standard_normal(n1))
+    M = rng.standard_normal((n2, n1 + 2))
+    got = host64(L * to_dev(M))
+    ref = po.mul_nd(oop(L), M, (n2, n1))
+    assert rel(got, ref) < 1e-12
+    # trimmed transverse padding + overwrite=false
+    M2 = rng.standard_normal((n2 + 2, n1 + 2))
+    du0 = rng.standard_normal((n2, n1))
+    du = to_dev(du0)
+    D.mul_(du, L, to_dev(M2), overwrite=False)
+    ref2 = po.mul_nd(oop(L), M2, (n2, n1)) + du0
+    assert rel(host64(du), ref2) < 1e-12
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_composite_mul_padded(nd):
+    rng = np.random.default_rng(6)
+    n = 40 if nd == 3 else 200
+    ops = [D.CenteredDifference[k](2, 4, 0.5, n) for k in range(1, nd + 1)]
+    ops.append(D.UpwindDifference[2](1, 2, 0.5, n, rng.standard_normal(n)))
+    A = ops[0]
+    for L in ops[1:]:
+        A = A + L
+    M = rng.standard_normal((n + 2,) * nd)
+    du = torch.empty((n,) * nd, dtype=torch.float64, device="cuda")
+    D.mul_(du, A, to_dev(M))
+    ref = po.composite_mul_padded([oop(L) for L in ops], M)
+    assert rel(host64(du), ref) < 1e-12
+
+
+def test_ghost_accumulate_and_bad_inputs():
+    n = 128
+    A, ops, Q = laplacian_robin(n, 1.0 / (n + 1))
+    rng = np.random.default_rng(8)
+    u = rng.uniform(-1, 1, (n, n))
+    du0 = rng.standard_normal((n, n))
+    du = to_dev(du0)
+    D.mul_(du, A, to_dev(u), overwrite=False)
+    ref = po.ghost_apply([oop(L) for L in ops], [obc(b.bc) for b in Q.bcs], u) + du0
+    assert rel(host64(du), ref) < 1e-12
+    ud = to_dev(u)
+    with pytest.raises(ValueError):
+        D.mul_(ud, A, ud)                         # aliasing
+    with pytest.raises(AssertionError):
+        A * to_dev(rng.standard_normal((n, n + 1)))   # BC sized for (n, n)
