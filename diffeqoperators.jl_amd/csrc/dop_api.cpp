@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,12 +41,11 @@ int device_grid(int per_cu)
 }  // namespace
 
 struct dop_plan {
-    dop::PlanDev host{};          // table pointers are device pointers
-    dop::PlanDev* dev = nullptr;  // device copy of host
+    dop::DevicePlan d;            // host copy (device table pointers), device copy, tiled path
+    dop::PlanDev* dev = nullptr;  // device copy of d.host
     std::vector<void*> bufs;
     int64_t in_elems = 0, out_elems = 0;
     int device = 0;
-    int variant = 0;
     ~dop_plan()
     {
         for (void* b : bufs) (void)hipFree(b);
@@ -121,13 +121,21 @@ int dop_plan_create(const dop_desc* desc, dop_plan** out)
     const int rc = dop::build_host_plan(*desc, &H, &err);
     if (rc != DOP_OK) return fail(rc, err);
     dop_plan* p = new dop_plan();
-    p->host = H.meta;
+    p->d.host = H.meta;
     p->in_elems = H.in_elems;
     p->out_elems = H.out_elems;
     (void)hipGetDevice(&p->device);
     int r = DOP_OK;
+    // DOP_FORCE_GENERIC=1 (read when a plan is created) keeps every point on the generic
+    // kernel: the reference for checking the tiled kernels bit for bit.
+    const char* fg = std::getenv("DOP_FORCE_GENERIC");
+    if (!(fg && fg[0] == '1')) dop::build_fast(H, &p->d.fast);
+    for (int a = 0; a < dop::kMaxDim && r == DOP_OK && p->d.fast.kind != 0; ++a) {
+        if ((r = upload(p, p->d.fast.w64[a], &p->d.fast.args.w64[a])) != DOP_OK) break;
+        if ((r = upload(p, p->d.fast.w32[a], &p->d.fast.args.w32[a])) != DOP_OK) break;
+    }
     for (int a = 0; a < dop::kMaxDim && r == DOP_OK; ++a) {
-        dop::AxisDev& X = p->host.ax[a];
+        dop::AxisDev& X = p->d.host.ax[a];
         const dop::AxisTables& T = H.axis[a];
         if ((r = upload(p, T.w64, &X.w64)) != DOP_OK) break;
         if ((r = upload(p, T.w32, &X.w32)) != DOP_OK) break;
@@ -138,10 +146,11 @@ int dop_plan_create(const dop_desc* desc, dop_plan** out)
     if (r == DOP_OK) {
         if (hipMalloc((void**)&p->dev, sizeof(dop::PlanDev)) != hipSuccess) {
             r = fail(DOP_ENOMEM, "hipMalloc of plan failed");
-        } else if (hipMemcpy(p->dev, &p->host, sizeof(dop::PlanDev), hipMemcpyHostToDevice) !=
+        } else if (hipMemcpy(p->dev, &p->d.host, sizeof(dop::PlanDev), hipMemcpyHostToDevice) !=
                    hipSuccess) {
             r = fail(DOP_EHIP, "upload of plan failed");
         }
+        p->d.dev = p->dev;
     }
     if (r != DOP_OK) {
         delete p;
@@ -160,21 +169,20 @@ int dop_apply(const dop_plan* plan, void* du, const void* u, int accumulate, voi
 {
     if (!plan || !du || !u) return fail(DOP_EINVAL, "null plan or buffer");
     if (du == u) return fail(DOP_EINVAL, "du must not alias u");
-    DOP_HIP(dop::launch_apply(plan->host, plan->dev, du, u, accumulate ? 1 : 0, plan->variant,
-                              (hipStream_t)stream));
+    DOP_HIP(dop::launch_apply(plan->d, du, u, accumulate ? 1 : 0, -1, 0, (hipStream_t)stream));
     return DOP_OK;
 }
 
 int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
 {
     if (!plan) return fail(DOP_EINVAL, "null plan");
-    const int nv = dop::apply_variants(plan->host);
+    const int nv = dop::apply_variants(plan->d);
     if (nv <= 1) {
-        plan->variant = 0;
+        plan->d.variant = 0;
         if (choice) *choice = 0;
         return DOP_OK;
     }
-    const size_t sb = storage_bytes(plan->host.storage);
+    const size_t sb = storage_bytes(plan->d.host.storage);
     void *du = nullptr, *u = nullptr;
     if (hipMalloc(&du, plan->out_elems * sb) != hipSuccess) return fail(DOP_ENOMEM, "autotune alloc");
     if (hipMalloc(&u, plan->in_elems * sb) != hipSuccess) {
@@ -193,7 +201,7 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
         bool ok = true;
         for (int rep = 0; rep < 4 && ok; ++rep) {
             (void)hipEventRecord(e0, s);
-            ok = dop::launch_apply(plan->host, plan->dev, du, u, 0, v, s) == hipSuccess;
+            ok = dop::launch_apply(plan->d, du, u, 0, v, 0, s) == hipSuccess;
             (void)hipEventRecord(e1, s);
             (void)hipEventSynchronize(e1);
             float ms = 0.0f;
@@ -209,7 +217,7 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
     (void)hipEventDestroy(e1);
     (void)hipFree(du);
     (void)hipFree(u);
-    plan->variant = best_v;
+    plan->d.variant = best_v;
     if (choice) *choice = best_v;
     return DOP_OK;
 }
@@ -221,17 +229,17 @@ int dop_batch_create(int n, const dop_plan* const* plans, void* const* du, const
     *out = nullptr;
     std::vector<dop::BatchItemDev> items(n);
     int64_t c = 0;
-    const int storage = plans[0] ? plans[0]->host.storage : 0;
+    const int storage = plans[0] ? plans[0]->d.host.storage : 0;
     for (int i = 0; i < n; ++i) {
         if (!plans[i] || !du[i] || !u[i]) return fail(DOP_EINVAL, "null plan or buffer in batch");
-        if (plans[i]->host.storage != storage) return fail(DOP_EINVAL, "mixed storage types in batch");
+        if (plans[i]->d.host.storage != storage) return fail(DOP_EINVAL, "mixed storage types in batch");
         if (du[i] == u[i]) return fail(DOP_EINVAL, "du must not alias u");
         items[i].plan = plans[i]->dev;
         items[i].du = du[i];
         items[i].u = u[i];
-        items[i].total = plans[i]->host.total;
+        items[i].total = plans[i]->d.host.total;
         items[i].chunk0 = c;
-        c += (plans[i]->host.total + dop::kChunkPts - 1) / dop::kChunkPts;
+        c += (plans[i]->d.host.total + dop::kChunkPts - 1) / dop::kChunkPts;
     }
     dop_batch* b = new dop_batch();
     b->n = n;
@@ -266,7 +274,7 @@ int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, con
 {
     if (!out || n < 1 || !plans || !scale || !dt || !u) return fail(DOP_EINVAL, "bad rk4 arguments");
     *out = nullptr;
-    const int storage = plans[0] ? plans[0]->host.storage : 0;
+    const int storage = plans[0] ? plans[0]->d.host.storage : 0;
     if (storage != DOP_F64 && storage != DOP_F32)
         return fail(DOP_EUNSUPPORTED, "rk4 needs f32 or f64 storage");
     dop_rk4* rk = new dop_rk4();
@@ -278,7 +286,7 @@ int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, con
     const size_t sb = storage_bytes(storage);
     for (int i = 0; i < n; ++i) {
         const dop_plan* p = plans[i];
-        if (!p || !u[i] || p->host.storage != storage || !p->host.ghost) {
+        if (!p || !u[i] || p->d.host.storage != storage || !p->d.host.ghost) {
             delete rk;
             return fail(DOP_EINVAL, "rk4: plans must be Q*u plans of one storage type");
         }
@@ -287,7 +295,7 @@ int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, con
         R.u = u[i];
         R.scale = scale[i];
         R.dt = dt[i];
-        R.total = p->host.total;
+        R.total = p->d.host.total;
         R.chunk0 = c;
         c += (R.total + dop::kChunkPts - 1) / dop::kChunkPts;
         void* bufs[3] = {nullptr, nullptr, nullptr};

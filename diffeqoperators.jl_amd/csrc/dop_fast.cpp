@@ -1,0 +1,130 @@
+// dop_fast.cpp -- host side of the tiled interior kernels: eligibility, interior box,
+// padded symmetric weights, shell decomposition and the tile-shape candidates.
+#include "dop_fast.hpp"
+
+#include <algorithm>
+
+namespace dop {
+
+namespace {
+
+constexpr int kMaxK2D = 13;
+constexpr int kMaxK3D = 9;   // 3-D stages 2*H halo rows through 8 waves: H <= 4
+constexpr int kWaves3D = 8;
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+void build_fast(const HostPlan& H, FastPlan* fp)
+{
+    *fp = FastPlan{};
+    const PlanDev& P = H.meta;
+    if (!P.ghost || (P.ndim != 2 && P.ndim != 3)) return;
+    const bool f64 = P.storage == kF64 && P.accum == kAccF64;
+    const bool f32 = (P.storage == kF32 || P.storage == kBF16) && P.accum == kAccF32;
+    if (!f64 && !f32) return;
+    // one symmetric window for all axes
+    int half = 0;
+    for (int a = 0; a < P.ndim; ++a) {
+        const AxisDev& X = P.ax[a];
+        if (!X.active || X.rhi <= X.rlo || X.K < 1) return;
+        half = std::max(half, std::max(-X.dmin, X.dmin + X.K - 1));
+    }
+    const int K = 2 * half + 1;
+    if (K > (P.ndim == 2 ? kMaxK2D : kMaxK3D)) return;
+    FastArgs& A = fp->args;
+    for (int a = 0; a < 3; ++a) A.n[a] = P.n[a];
+    A.s1 = P.sdu[1];
+    A.s2 = P.sdu[2];
+    A.K = K;
+    for (int a = 0; a < 3; ++a) {
+        if (a >= P.ndim) { A.lo[a] = 0; A.hi[a] = 1; continue; }
+        const AxisDev& X = P.ax[a];
+        A.lo[a] = std::max(X.rlo, half);
+        A.hi[a] = std::min(X.rhi, (int)P.n[a] - half);
+        if (A.hi[a] <= A.lo[a]) return;
+    }
+    // points per lane: 16-byte lane loads when rows stay aligned
+    const int vx_native = (P.storage == kF64) ? 2 : 4;
+    fp->VX = (P.n[0] % vx_native == 0) ? vx_native : 1;
+    // padded weights (rows outside the box are zero)
+    for (int a = 0; a < P.ndim; ++a) {
+        const AxisDev& X = P.ax[a];
+        const AxisTables& T = H.axis[a];
+        const int rows = X.uniform ? 1 : (int)P.n[a];
+        fp->w64[a].assign((size_t)rows * K, 0.0);
+        for (int r = 0; r < rows; ++r) {
+            const int src = X.uniform ? 0 : r;
+            if (!X.uniform && (r < A.lo[a] || r >= A.hi[a])) continue;
+            for (int k = 0; k < X.K; ++k)
+                fp->w64[a][(size_t)r * K + (X.dmin + k + half)] = T.w64[(size_t)src * X.K + k];
+        }
+        fp->w32[a].resize(fp->w64[a].size());
+        for (size_t i = 0; i < fp->w64[a].size(); ++i) fp->w32[a][i] = (float)fp->w64[a][i];
+        A.uniform[a] = X.uniform;
+    }
+    A.xbase = (A.lo[0] / fp->VX) * fp->VX;
+    A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, 64 * fp->VX);
+    // shell: the domain minus the box, slabs from the slowest axis down
+    ShellArgs& S = fp->shell;
+    S.nbox = 0;
+    int64_t total = 0;
+    for (int a = P.ndim - 1; a >= 0; --a) {
+        for (int side = 0; side < 2; ++side) {
+            int64_t org[3], ext[3];
+            for (int b = 0; b < 3; ++b) {
+                if (b > a) { org[b] = A.lo[b]; ext[b] = A.hi[b] - A.lo[b]; }   // inside box
+                else if (b < a) { org[b] = 0; ext[b] = P.n[b]; }               // full
+                else if (side == 0) { org[b] = 0; ext[b] = A.lo[b]; }
+                else { org[b] = A.hi[b]; ext[b] = P.n[b] - A.hi[b]; }
+            }
+            const int64_t cnt = ext[0] * ext[1] * ext[2];
+            if (cnt <= 0) continue;
+            for (int b = 0; b < 3; ++b) {
+                S.org[S.nbox][b] = org[b];
+                S.ext[S.nbox][b] = ext[b];
+            }
+            S.start[S.nbox] = total;
+            total += cnt;
+            ++S.nbox;
+        }
+    }
+    S.start[S.nbox] = total;
+    fp->shell_total = total;
+    fp->kind = P.ndim;
+}
+
+int fast_variant_count(const FastPlan& fp)
+{
+    if (fp.kind == 2) return 8;   // 4 chunk heights x XCD remap on/off
+    if (fp.kind == 3) return 6;   // 3 chunk depths x XCD remap on/off
+    return 1;
+}
+
+bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
+{
+    if (fp.kind == 0 || v < 0 || v >= fast_variant_count(fp)) return false;
+    FastArgs A = fp.args;
+    A.xcd_remap = (v & 1) ? 0 : 1;
+    const int c = v >> 1;
+    if (fp.kind == 2) {
+        static const int chunks[4] = {32, 16, 64, 128};
+        A.chunk = chunks[c];
+        A.gx = (int)ceil_div(A.nxseg, 4);
+        const int64_t nch = ceil_div(A.hi[1] - A.lo[1], A.chunk);
+        A.nyblk = 0;
+        A.nblocks = (int)(A.gx * nch);
+    } else {
+        static const int chunks[3] = {64, 32, 128};
+        A.chunk = chunks[c];
+        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], kWaves3D);
+        const int64_t nzc = ceil_div(A.hi[2] - A.lo[2], A.chunk);
+        A.gx = 0;
+        A.nblocks = (int)(A.nxseg * A.nyblk * nzc);
+    }
+    *out = A;
+    return true;
+}
+
+}  // namespace dop

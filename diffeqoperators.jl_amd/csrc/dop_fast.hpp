@@ -1,0 +1,62 @@
+// dop_fast.hpp -- launch descriptions of the tiled interior kernels (host-built, passed by
+// value to the kernels) and of the generic "shell" launch that covers the rest.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "dop_build.hpp"
+#include "dop_plan.hpp"
+
+namespace dop {
+
+constexpr int kMaxBoxes = 6;
+
+// Points outside the interior box, as up to 6 axis-aligned boxes.
+struct ShellArgs {
+    int32_t nbox;
+    int32_t pad_;
+    int64_t org[kMaxBoxes][3];
+    int64_t ext[kMaxBoxes][3];
+    int64_t start[kMaxBoxes + 1];   // prefix counts; start[nbox] = total
+};
+
+// The interior box and the per-axis weights of the tiled kernels.  Every active axis uses
+// one symmetric odd window of K taps (narrower operator windows are padded with zero
+// weights); rows/planes march with a register ring.
+struct FastArgs {
+    int64_t n[3];
+    int64_t s1, s2;        // row / plane strides
+    int32_t K;             // window taps on every axis
+    int32_t lo[3], hi[3];  // interior box [lo, hi) per axis
+    int32_t xbase;         // aligned x of the first segment
+    int32_t nxseg;         // x segments (64*VX points) per row
+    int32_t chunk;         // rows (2-D) / planes (3-D) per block
+    int32_t nyblk;         // 3-D: y blocks of W rows
+    int32_t gx;            // 2-D: blocks across x (4 segments each)
+    int32_t nblocks;
+    int32_t xcd_remap;     // keep neighbouring tiles on one XCD
+    int32_t uniform[3];    // weights identical for every row of the axis
+    const double* w64[3];  // [uniform ? 1 : n][K]
+    const float* w32[3];
+};
+
+struct FastPlan {
+    int kind = 0;          // 0: generic only, 2: 2-D tiled, 3: 3-D tiled
+    int VX = 1;            // points per lane along x
+    FastArgs args{};
+    ShellArgs shell{};
+    int64_t shell_total = 0;
+    std::vector<double> w64[3];   // host copies before upload
+    std::vector<float> w32[3];
+};
+
+// Decides whether the plan has a tiled path and builds its description (host tables in
+// fp->w64/w32; device pointers are filled by the caller after upload).
+void build_fast(const HostPlan& H, FastPlan* fp);
+
+// Launch-time geometry for variant `v` (autotuner candidates); returns false if `v` is
+// not a valid variant.
+bool fast_variant(const FastPlan& fp, int v, FastArgs* out);
+int fast_variant_count(const FastPlan& fp);
+
+}  // namespace dop

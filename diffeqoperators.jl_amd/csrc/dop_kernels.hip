@@ -47,16 +47,14 @@ static hipError_t launch_generic(const PlanDev& host, const PlanDev* dev, void* 
     return hipGetLastError();
 }
 
-int apply_variants(const PlanDev& host)
-{
-    (void)host;
-    return 1;
-}
+int apply_variants(const DevicePlan& p) { return fast_variant_count(p.fast); }
 
-hipError_t launch_apply(const PlanDev& host, const PlanDev* dev, void* du, const void* u,
-                        int accumulate, int variant, hipStream_t s)
+hipError_t launch_tiled(const DevicePlan& p, void* du, const void* u, int accumulate, int variant,
+                        hipStream_t s);
+
+static hipError_t launch_generic_any(const PlanDev& host, const PlanDev* dev, void* du,
+                                     const void* u, int accumulate, hipStream_t s)
 {
-    (void)variant;
     switch (host.storage) {
     case kF64:
         return launch_generic<double, kAccF64>(host, dev, du, u, accumulate, s);
@@ -72,6 +70,14 @@ hipError_t launch_apply(const PlanDev& host, const PlanDev* dev, void* du, const
         return launch_generic<bf16, kAccF32>(host, dev, du, u, accumulate, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumulate,
+                        int variant, int force_generic, hipStream_t s)
+{
+    if (p.fast.kind != 0 && !force_generic)
+        return launch_tiled(p, du, u, accumulate, variant < 0 ? p.variant : variant, s);
+    return launch_generic_any(p.host, p.dev, du, u, accumulate, s);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -2,9 +2,19 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "dop_fast.hpp"
 #include "dop_plan.hpp"
 
 namespace dop {
+
+// Everything a launch needs: the host copy of the plan (table pointers are device
+// pointers), its device copy, and the tiled-path description.
+struct DevicePlan {
+    PlanDev host{};
+    const PlanDev* dev = nullptr;
+    FastPlan fast;
+    int variant = 0;   // tile-shape candidate chosen by the autotuner
+};
 
 // One work item of a persistent batched launch: a plan bound to its buffers.
 struct BatchItemDev {
@@ -31,12 +41,13 @@ struct RkItemDev {
 constexpr int kBlock = 256;        // threads per block
 constexpr int kChunkPts = 1024;    // points per chunk of a persistent launch
 
-// Single-plan apply.  `host` is the host copy of *dev (used for launch decisions).
-hipError_t launch_apply(const PlanDev& host, const PlanDev* dev, void* du, const void* u,
-                        int accumulate, int variant, hipStream_t s);
+// Single-plan apply: the tiled interior + shell when the plan has one, else generic.
+// variant < 0 uses the plan's tuned variant; force_generic = 1 runs the generic kernel.
+hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumulate,
+                        int variant, int force_generic, hipStream_t s);
 
-// Number of kernel variants launch_apply can choose from for this plan (autotuner).
-int apply_variants(const PlanDev& host);
+// Number of tile-shape candidates of the plan (autotuner).
+int apply_variants(const DevicePlan& p);
 
 hipError_t launch_batch(int storage, const BatchItemDev* items, int n, int64_t nchunks,
                         int accumulate, int grid, hipStream_t s);
