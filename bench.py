@@ -82,6 +82,7 @@ def bench_laplacian2d(steps, warmup):
     n = 4096
     A, u, _, _ = laplacian2d(n)
     plan = D.plans_for(A, u)[0]
+    variant = plan.autotune()
     du = torch.empty_like(u)
     wall, ev = time_applies(lambda: plan.apply(du, u), steps, warmup)
     pts = n * n
@@ -93,7 +94,7 @@ def bench_laplacian2d(steps, warmup):
         roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(achieved / HBM_PEAK_GBS, 4),
                       traffic=read_traffic("apply_laplacian2d_4096_f64")),
-        alg_bytes=alg_bytes)
+        alg_bytes=alg_bytes, variant=variant)
 
 
 def cpu_baseline_laplacian2d(seconds):
@@ -147,13 +148,15 @@ def bench_nonuniform3d(steps, warmup, n=512):
     for name, dt, sb in (("f64", torch.float64, 8), ("f32", torch.float32, 4), ("bf16", torch.bfloat16, 2)):
         A, u = nonuniform3d(n, dt)
         plan = D.plans_for(A, u)[0]
+        variant = plan.autotune()
         du = torch.empty_like(u)
         wall, ev = time_applies(lambda: plan.apply(du, u), steps, warmup)
         pts = n ** 3
         kern = ev / steps
         ach = 2 * sb * pts / kern / 1e9
         out[name] = dict(gpts_s=round(pts / kern / 1e9, 2), kernel_ms=round(kern * 1e3, 4),
-                         achieved_GBs=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4))
+                         achieved_GBs=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4),
+                         tile_variant=variant)
         del A, u, du, plan
         torch.cuda.empty_cache()
     return out
@@ -189,7 +192,8 @@ def main():
             "data": "synthetic (seeded uniform [-1,1] field)",
             "config": {"workload": "2-D Laplacian CenteredDifference{1,2}(2,4,h,4096) + "
                                    "RobinBC(1,0.5,0) on all edges, 4096x4096 FP64, one Δ*Q*u per step",
-                       "grid": [4096, 4096], "storage": "f64", "accumulation": "f64"},
+                       "grid": [4096, 4096], "storage": "f64", "accumulation": "f64",
+                       "tile_variant": r["variant"]},
             "roofline": r["roofline"],
         }
     else:

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -9,6 +10,7 @@
 
 #include "../../include/dop.h"
 #include "dop_build.hpp"
+#include "dop_batch.hpp"
 #include "dop_kernels.hpp"
 
 namespace {
@@ -53,24 +55,26 @@ struct dop_plan {
     }
 };
 
+// A batch (apply or RK4): problem table + work list on the device, host copies kept for
+// re-chunking by the autotuner.
 struct dop_batch {
-    dop::BatchItemDev* items = nullptr;
-    int n = 0;
-    int64_t nchunks = 0;
+    std::vector<dop::BProb> probs;
+    std::vector<dop::BItem> items;
+    dop::BProb* d_probs = nullptr;
+    dop::BItem* d_items = nullptr;
+    int nitems = 0;
     int storage = 0;
     int grid = 0;
+    int chunk = 16;
     ~dop_batch()
     {
-        if (items) (void)hipFree(items);
+        if (d_probs) (void)hipFree(d_probs);
+        if (d_items) (void)hipFree(d_items);
     }
 };
 
 struct dop_rk4 {
-    dop::RkItemDev* items = nullptr;
-    int n = 0;
-    int64_t nchunks = 0;
-    int storage = 0;
-    int grid = 0;
+    dop_batch b;
     std::vector<void*> bufs;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -78,8 +82,7 @@ struct dop_rk4 {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* b : bufs) (void)hipFree(b);
-        if (items) (void)hipFree(items);
+        for (void* p : bufs) (void)hipFree(p);
     }
 };
 
@@ -130,6 +133,11 @@ int dop_plan_create(const dop_desc* desc, dop_plan** out)
     // kernel: the reference for checking the tiled kernels bit for bit.
     const char* fg = std::getenv("DOP_FORCE_GENERIC");
     if (!(fg && fg[0] == '1')) dop::build_fast(H, &p->d.fast);
+    // DOP_TILE_VARIANT=v pins the tile-shape candidate (otherwise 0 until dop_plan_autotune)
+    if (const char* tv = std::getenv("DOP_TILE_VARIANT")) {
+        const int v = std::atoi(tv);
+        if (v >= 0 && v < dop::apply_variants(p->d)) p->d.variant = v;
+    }
     for (int a = 0; a < dop::kMaxDim && r == DOP_OK && p->d.fast.kind != 0; ++a) {
         if ((r = upload(p, p->d.fast.w64[a], &p->d.fast.args.w64[a])) != DOP_OK) break;
         if ((r = upload(p, p->d.fast.w32[a], &p->d.fast.args.w32[a])) != DOP_OK) break;
@@ -196,6 +204,8 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
     (void)hipEventCreate(&e1);
     float best = 1e30f;
     int best_v = 0;
+    const char* vb = std::getenv("DOP_AUTOTUNE_VERBOSE");
+    const bool verbose = vb && vb[0] == '1';
     for (int v = 0; v < nv; ++v) {
         float tsum = 0.0f;
         bool ok = true;
@@ -208,6 +218,9 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
             (void)hipEventElapsedTime(&ms, e0, e1);
             if (rep > 0) tsum += ms;
         }
+        if (verbose)
+            fprintf(stderr, "dop_plan_autotune: variant %d %s %.4f ms\n", v, ok ? "ok" : "failed",
+                    tsum / 3.0f);
         if (ok && tsum < best) {
             best = tsum;
             best_v = v;
@@ -222,38 +235,116 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
     return DOP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// (re)builds the work list for row chunk `chunk` and uploads probs + items
+int batch_upload(dop_batch* b, int chunk)
+{
+    for (auto& P : b->probs) P.chunk = chunk;
+    dop::batch_items(b->probs, &b->items);
+    b->chunk = chunk;
+    b->nitems = (int)b->items.size();
+    if (!b->d_probs) {
+        if (hipMalloc((void**)&b->d_probs, sizeof(dop::BProb) * b->probs.size()) != hipSuccess)
+            return fail(DOP_ENOMEM, "hipMalloc of batch problem table failed");
+    }
+    if (b->d_items) (void)hipFree(b->d_items);
+    b->d_items = nullptr;
+    if (hipMalloc((void**)&b->d_items, sizeof(dop::BItem) * std::max<size_t>(b->items.size(), 1)) !=
+        hipSuccess)
+        return fail(DOP_ENOMEM, "hipMalloc of batch work list failed");
+    DOP_HIP(hipMemcpy(b->d_probs, b->probs.data(), sizeof(dop::BProb) * b->probs.size(),
+                      hipMemcpyHostToDevice));
+    DOP_HIP(hipMemcpy(b->d_items, b->items.data(), sizeof(dop::BItem) * b->items.size(),
+                      hipMemcpyHostToDevice));
+    return DOP_OK;
+}
+
+int batch_init(dop_batch* b, int n, const dop_plan* const* plans)
+{
+    b->storage = plans[0]->d.host.storage;
+    b->grid = device_grid(8);
+    b->probs.resize(n);
+    for (int i = 0; i < n; ++i) {
+        if (plans[i]->d.host.storage != b->storage)
+            return fail(DOP_EINVAL, "mixed storage types in batch");
+        dop::BProb& P = b->probs[i];
+        P = dop::BProb{};
+        dop::batch_geometry(plans[i]->d, b->chunk, &P);
+    }
+    return DOP_OK;
+}
+
+hipError_t batch_launch(const dop_batch* b, int stage, int accumulate, hipStream_t s)
+{
+    return dop::launch_batch_items(b->storage, b->d_probs, b->d_items, b->nitems, stage,
+                                   accumulate, b->grid, s);
+}
+
+// times `stage` launches for each candidate chunk height and keeps the fastest
+int batch_tune(dop_batch* b, int stage, hipStream_t s, int* choice)
+{
+    static const int cands[4] = {16, 32, 8, 64};
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const char* vb = std::getenv("DOP_AUTOTUNE_VERBOSE");
+    const bool verbose = vb && vb[0] == '1';
+    float best = 1e30f;
+    int best_c = 0, r = DOP_OK;
+    for (int c = 0; c < 4 && r == DOP_OK; ++c) {
+        if ((r = batch_upload(b, cands[c])) != DOP_OK) break;
+        float tsum = 0.0f;
+        for (int rep = 0; rep < 4; ++rep) {
+            (void)hipEventRecord(e0, s);
+            if (batch_launch(b, stage, 0, s) != hipSuccess) {
+                r = fail(DOP_EHIP, "batched launch failed during tuning");
+                break;
+            }
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) tsum += ms;
+        }
+        if (verbose) fprintf(stderr, "batch autotune: chunk %d %.4f ms\n", cands[c], tsum / 3.0f);
+        if (tsum < best) {
+            best = tsum;
+            best_c = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r != DOP_OK) return r;
+    if (choice) *choice = cands[best_c];
+    return batch_upload(b, cands[best_c]);
+}
+
+}  // namespace
+
+extern "C" {
+
 int dop_batch_create(int n, const dop_plan* const* plans, void* const* du, const void* const* u,
                      dop_batch** out)
 {
     if (!out || n < 1 || !plans || !du || !u) return fail(DOP_EINVAL, "bad batch arguments");
     *out = nullptr;
-    std::vector<dop::BatchItemDev> items(n);
-    int64_t c = 0;
-    const int storage = plans[0] ? plans[0]->d.host.storage : 0;
     for (int i = 0; i < n; ++i) {
         if (!plans[i] || !du[i] || !u[i]) return fail(DOP_EINVAL, "null plan or buffer in batch");
-        if (plans[i]->d.host.storage != storage) return fail(DOP_EINVAL, "mixed storage types in batch");
         if (du[i] == u[i]) return fail(DOP_EINVAL, "du must not alias u");
-        items[i].plan = plans[i]->dev;
-        items[i].du = du[i];
-        items[i].u = u[i];
-        items[i].total = plans[i]->d.host.total;
-        items[i].chunk0 = c;
-        c += (plans[i]->d.host.total + dop::kChunkPts - 1) / dop::kChunkPts;
     }
     dop_batch* b = new dop_batch();
-    b->n = n;
-    b->nchunks = c;
-    b->storage = storage;
-    b->grid = device_grid(8);
-    if (hipMalloc((void**)&b->items, sizeof(dop::BatchItemDev) * n) != hipSuccess) {
-        delete b;
-        return fail(DOP_ENOMEM, "hipMalloc of batch failed");
+    int r = batch_init(b, n, plans);
+    for (int i = 0; i < n && r == DOP_OK; ++i) {
+        b->probs[i].src = u[i];
+        b->probs[i].out = du[i];
     }
-    if (hipMemcpy(b->items, items.data(), sizeof(dop::BatchItemDev) * n, hipMemcpyHostToDevice) !=
-        hipSuccess) {
+    if (r == DOP_OK) r = batch_upload(b, b->chunk);
+    if (r != DOP_OK) {
         delete b;
-        return fail(DOP_EHIP, "upload of batch failed");
+        return r;
     }
     *out = b;
     return DOP_OK;
@@ -262,9 +353,14 @@ int dop_batch_create(int n, const dop_plan* const* plans, void* const* du, const
 int dop_batch_apply(const dop_batch* batch, int accumulate, void* stream)
 {
     if (!batch) return fail(DOP_EINVAL, "null batch");
-    DOP_HIP(dop::launch_batch(batch->storage, batch->items, batch->n, batch->nchunks,
-                              accumulate ? 1 : 0, batch->grid, (hipStream_t)stream));
+    DOP_HIP(batch_launch(batch, -1, accumulate ? 1 : 0, (hipStream_t)stream));
     return DOP_OK;
+}
+
+int dop_batch_autotune(dop_batch* batch, void* stream, int* choice)
+{
+    if (!batch) return fail(DOP_EINVAL, "null batch");
+    return batch_tune(batch, -1, (hipStream_t)stream, choice);
 }
 
 void dop_batch_destroy(dop_batch* batch) { delete batch; }
@@ -274,73 +370,57 @@ int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, con
 {
     if (!out || n < 1 || !plans || !scale || !dt || !u) return fail(DOP_EINVAL, "bad rk4 arguments");
     *out = nullptr;
-    const int storage = plans[0] ? plans[0]->d.host.storage : 0;
+    for (int i = 0; i < n; ++i)
+        if (!plans[i] || !u[i] || !plans[i]->d.host.ghost)
+            return fail(DOP_EINVAL, "rk4: every problem needs a Q*u plan and a state buffer");
+    const int storage = plans[0]->d.host.storage;
     if (storage != DOP_F64 && storage != DOP_F32)
         return fail(DOP_EUNSUPPORTED, "rk4 needs f32 or f64 storage");
     dop_rk4* rk = new dop_rk4();
-    rk->n = n;
-    rk->storage = storage;
-    rk->grid = device_grid(8);
-    std::vector<dop::RkItemDev> items(n);
-    int64_t c = 0;
+    int r = batch_init(&rk->b, n, plans);
     const size_t sb = storage_bytes(storage);
-    for (int i = 0; i < n; ++i) {
-        const dop_plan* p = plans[i];
-        if (!p || !u[i] || p->d.host.storage != storage || !p->d.host.ghost) {
-            delete rk;
-            return fail(DOP_EINVAL, "rk4: plans must be Q*u plans of one storage type");
-        }
-        dop::RkItemDev& R = items[i];
-        R.plan = p->dev;
-        R.u = u[i];
-        R.scale = scale[i];
-        R.dt = dt[i];
-        R.total = p->d.host.total;
-        R.chunk0 = c;
-        c += (R.total + dop::kChunkPts - 1) / dop::kChunkPts;
+    for (int i = 0; i < n && r == DOP_OK; ++i) {
+        dop::BProb& P = rk->b.probs[i];
+        P.u = u[i];
+        P.scale = scale[i];
+        P.dt = dt[i];
         void* bufs[3] = {nullptr, nullptr, nullptr};
-        for (int k = 0; k < 3; ++k) {
-            if (hipMalloc(&bufs[k], R.total * sb) != hipSuccess) {
-                delete rk;
-                return fail(DOP_ENOMEM, "rk4: stage buffer allocation failed");
-            }
-            rk->bufs.push_back(bufs[k]);
+        for (int k = 0; k < 3 && r == DOP_OK; ++k) {
+            if (hipMalloc(&bufs[k], plans[i]->d.host.total * sb) != hipSuccess)
+                r = fail(DOP_ENOMEM, "rk4: stage buffer allocation failed");
+            else
+                rk->bufs.push_back(bufs[k]);
         }
-        R.s1 = bufs[0];
-        R.s2 = bufs[1];
-        R.acc = bufs[2];
+        P.s1 = bufs[0];
+        P.s2 = bufs[1];
+        P.acc = bufs[2];
     }
-    rk->nchunks = c;
-    if (hipMalloc((void**)&rk->items, sizeof(dop::RkItemDev) * n) != hipSuccess) {
-        delete rk;
-        return fail(DOP_ENOMEM, "rk4: item table allocation failed");
-    }
-    if (hipMemcpy(rk->items, items.data(), sizeof(dop::RkItemDev) * n, hipMemcpyHostToDevice) !=
-        hipSuccess) {
-        delete rk;
-        return fail(DOP_EHIP, "rk4: item upload failed");
+    hipStream_t s = (hipStream_t)stream;
+    // tile shape: stage 0 only reads u and writes acc / s1, so it can be timed repeatedly
+    const char* at = std::getenv("DOP_RK4_AUTOTUNE");
+    if (r == DOP_OK) {
+        if (at && at[0] == '0') r = batch_upload(&rk->b, rk->b.chunk);
+        else r = batch_tune(&rk->b, 0, s, nullptr);
     }
     // capture one step (four stage launches) into a graph
     hipStream_t cs = nullptr;
-    int r = DOP_OK;
-    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
-        delete rk;
-        return fail(DOP_EHIP, "rk4: stream creation failed");
+    if (r == DOP_OK && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+        r = fail(DOP_EHIP, "rk4: stream creation failed");
+    if (r == DOP_OK) {
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            r = fail(DOP_EHIP, "rk4: begin capture failed");
+        } else {
+            for (int st = 0; st < 4; ++st)
+                if (batch_launch(&rk->b, st, 0, cs) != hipSuccess)
+                    r = fail(DOP_EHIP, "rk4: stage launch during capture failed");
+            if (hipStreamEndCapture(cs, &rk->graph) != hipSuccess && r == DOP_OK)
+                r = fail(DOP_EHIP, "rk4: end capture failed");
+            if (r == DOP_OK &&
+                hipGraphInstantiate(&rk->exec, rk->graph, nullptr, nullptr, 0) != hipSuccess)
+                r = fail(DOP_EHIP, "rk4: graph instantiation failed");
+        }
     }
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        r = fail(DOP_EHIP, "rk4: begin capture failed");
-    } else {
-        for (int st = 0; st < 4; ++st)
-            if (dop::launch_rk4_stage(storage, rk->items, n, rk->nchunks, st, rk->grid, cs) !=
-                hipSuccess)
-                r = fail(DOP_EHIP, "rk4: stage launch during capture failed");
-        if (hipStreamEndCapture(cs, &rk->graph) != hipSuccess && r == DOP_OK)
-            r = fail(DOP_EHIP, "rk4: end capture failed");
-        if (r == DOP_OK && hipGraphInstantiate(&rk->exec, rk->graph, nullptr, nullptr, 0) != hipSuccess)
-            r = fail(DOP_EHIP, "rk4: graph instantiation failed");
-    }
-    (void)hipStreamDestroy(cs);
-    (void)stream;
+    if (cs) (void)hipStreamDestroy(cs);
     if (r != DOP_OK) {
         delete rk;
         return r;
@@ -355,6 +435,8 @@ int dop_rk4_step(dop_rk4* rk, int nsteps, void* stream)
     for (int i = 0; i < nsteps; ++i) DOP_HIP(hipGraphLaunch(rk->exec, (hipStream_t)stream));
     return DOP_OK;
 }
+
+int dop_rk4_chunk(const dop_rk4* rk) { return rk ? rk->b.chunk : -1; }
 
 void dop_rk4_destroy(dop_rk4* rk) { delete rk; }
 

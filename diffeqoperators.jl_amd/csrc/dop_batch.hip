@@ -1,0 +1,268 @@
+// dop_batch.hip -- one persistent launch over many independent problems.
+//
+// The host turns a batch into a work list: every 2-D problem with a tiled path contributes
+// row-chunk tiles (4 wave segments x `chunk` rows) plus chunks of its shell points; every
+// other problem (1-D, 3-D, ...) contributes chunks of points for the generic evaluator.
+// Blocks walk the list with a static stride.  A 2-D tile keeps one point per lane and
+// takes the x window from neighbouring lanes with DPP wavefront shifts (wave_shr/wave_shl
+// by one lane per step), so a wave's 64 lanes yield 64-2H outputs per row; the y window
+// comes from a register ring with PF rows of prefetch, as in the single-plan kernel.
+// Every path sums a point's taps in the generic order with the plan's accumulator (f64,
+// f32 or compensated f32), so batched and single results agree bitwise.
+//
+// Epilogues: plain apply (du = L Q u, optionally += du), or one stage of classical RK4.
+#include <hip/hip_runtime.h>
+
+#include "dop_batch.hpp"
+#include "dop_eval.hpp"
+
+namespace dop {
+
+namespace {
+
+// lane i <- lane i-1 (wave_shr:1) / lane i <- lane i+1 (wave_shl:1); edge lanes get 0
+__device__ inline float shr1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                  0x138, 0xf, 0xf, false));
+}
+__device__ inline float shl1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                  0x130, 0xf, 0xf, false));
+}
+__device__ inline double shr1(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ inline double shl1(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+template <typename T>
+struct StageBufs {
+    const T* X;   // stencil input
+    T* out;       // apply: du
+    T* u;
+    T* acc;
+    T* nxt;
+};
+
+// store of one point: plain apply, or RK4 stage `stage` (see dop_kernels.hip)
+template <typename T>
+__device__ inline void epilogue(const BProb& B, const StageBufs<T>& S, int stage, int accumulate,
+                                int64_t p, double v)
+{
+    if (stage < 0) {
+        if (accumulate) v += (double)Store<T>::ld(S.out, p);
+        Store<T>::st(S.out, p, v);
+        return;
+    }
+    const double k = B.scale * v;
+    const double dt = B.dt;
+    if (stage == 3) {
+        Store<T>::st(S.u, p, (double)Store<T>::ld(S.acc, p) + (dt / 6.0) * k);
+    } else {
+        const double up = (double)Store<T>::ld(S.u, p);
+        if (stage == 0) Store<T>::st(S.acc, p, up + (dt / 6.0) * k);
+        else Store<T>::st(S.acc, p, (double)Store<T>::ld(S.acc, p) + (dt / 3.0) * k);
+        const double h = (stage == 2) ? dt : 0.5 * dt;
+        Store<T>::st(S.nxt, p, up + h * k);
+    }
+}
+
+template <typename T>
+__device__ inline StageBufs<T> stage_bufs(const BProb& B, int stage)
+{
+    StageBufs<T> S;
+    S.out = (T*)B.out;
+    S.u = (T*)B.u;
+    S.acc = (T*)B.acc;
+    if (stage < 0) {
+        S.X = (const T*)B.src;
+        S.nxt = nullptr;
+    } else {
+        S.X = stage == 0 ? (const T*)B.u : stage == 1 ? (const T*)B.s1
+            : stage == 2 ? (const T*)B.s2 : (const T*)B.s1;
+        S.nxt = stage == 1 ? (T*)B.s2 : (T*)B.s1;
+    }
+    return S;
+}
+
+template <int ACC>
+__device__ inline const typename Acc<ACC>::W* btab(const BProb& B, int a);
+template <>
+__device__ inline const double* btab<kAccF64>(const BProb& B, int a) { return B.w64[a]; }
+template <>
+__device__ inline const float* btab<kAccF32>(const BProb& B, int a) { return B.w32[a]; }
+template <>
+__device__ inline const float* btab<kAccF32Kahan>(const BProb& B, int a) { return B.w32[a]; }
+
+// ---------------------------------------------------------------------------------------
+// 2-D tile: wave segment `seg` (64 lanes, 64-2H outputs), rows [y0, y1)
+template <typename T, int ACC, int K>
+__device__ void tile2d(const BProb& B, const StageBufs<T>& S, int stage, int accumulate, int xg,
+                       int ch)
+{
+    using W = typename Acc<ACC>::W;
+    constexpr int H = K / 2;
+    constexpr int PF = 2;
+    constexpr int R = K + PF;
+    constexpr int OUTW = 64 - 2 * H;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int seg = xg * 4 + wave;
+    if (seg >= B.nseg) return;
+    const int y0 = B.lo[1] + ch * B.chunk;
+    const int y1 = min(y0 + B.chunk, B.hi[1]);
+    const int64_t n0 = B.n0;
+    const int n1 = (int)B.n1;
+    const int64_t s1 = B.ys;
+    const int64_t x = (int64_t)B.lo[0] + (int64_t)seg * OUTW + (lane - H);   // this lane's column
+    const bool xin = x >= 0 && x < n0;
+    const bool out_ok = lane >= H && lane < 64 - H && x < B.hi[0];
+    const W* wxt = btab<ACC>(B, 0);
+    const W* wyt = btab<ACC>(B, 1);
+    W wx[K];
+    {
+        const int64_t xr = B.uniform[0] ? 0 : min(max(x, (int64_t)0), n0 - 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) wx[k] = wxt[xr * K + k];
+    }
+    W q[R];
+    auto load_row = [&](int row) -> W {
+        row = min(max(row, 0), n1 - 1);
+        return xin ? (W)Store<T>::ld(S.X, (int64_t)row * s1 + x) : W(0);
+    };
+#pragma unroll
+    for (int s = 0; s < R - 1; ++s) q[s] = load_row(y0 - H + s);
+    for (int yb = y0; yb < y1; yb += R) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int y = yb + j;
+            if (y >= y1) break;
+            q[(j + K - 1 + PF) % R] = load_row(y + H + PF);
+            // x window of the centre row from the neighbouring lanes
+            W xw[K];
+            const W c = q[(j + H) % R];
+            xw[H] = c;
+            W t = c;
+#pragma unroll
+            for (int m = 1; m <= H; ++m) {
+                t = shl1(t);
+                xw[H + m] = t;
+            }
+            t = c;
+#pragma unroll
+            for (int m = 1; m <= H; ++m) {
+                t = shr1(t);
+                xw[H - m] = t;
+            }
+            const int64_t yr = B.uniform[1] ? 0 : y;
+            Acc<ACC> acc;
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc.add(wx[k], xw[k]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc.add(wyt[yr * K + k], q[(j + k) % R]);
+            if (out_ok) epilogue<T>(B, S, stage, accumulate, (int64_t)y * s1 + x, acc.value());
+        }
+    }
+}
+
+// generic chunk: points [c*kChunkPts, ...) of the problem's shell boxes
+template <typename T, int ACC>
+__device__ void shell_chunk(const BProb& B, const StageBufs<T>& S, int stage, int accumulate,
+                            int64_t c)
+{
+    const PlanDev& P = *B.plan;
+    const ShellArgs& SH = B.shell;
+    const int64_t total = SH.start[SH.nbox];
+    const int64_t first = c * kChunkPts;
+    for (int j = threadIdx.x; j < kChunkPts; j += kBlock) {
+        const int64_t p = first + j;
+        if (p >= total) break;
+        int b = 0;
+        while (b + 1 < SH.nbox && p >= SH.start[b + 1]) ++b;
+        int64_t l = p - SH.start[b];
+        const int64_t e0 = SH.ext[b][0], e1 = SH.ext[b][1];
+        int64_t idx[3];
+        idx[0] = SH.org[b][0] + l % e0;
+        l /= e0;
+        idx[1] = SH.org[b][1] + l % e1;
+        idx[2] = SH.org[b][2] + l / e1;
+        const int64_t lin = idx[0] + idx[1] * P.sdu[1] + idx[2] * P.sdu[2];
+        epilogue<T>(B, S, stage, accumulate, lin, eval_point<T, ACC>(P, S.X, idx));
+    }
+}
+
+template <typename T, int ACC>
+__device__ void run_item(const BProb& B, const BItem& I, int stage, int accumulate)
+{
+    const StageBufs<T> S = stage_bufs<T>(B, stage);
+    if (I.kind == 1) {
+        shell_chunk<T, ACC>(B, S, stage, accumulate, I.a);
+        return;
+    }
+    switch (B.K) {
+    case 3: tile2d<T, ACC, 3>(B, S, stage, accumulate, I.a, I.b); break;
+    case 5: tile2d<T, ACC, 5>(B, S, stage, accumulate, I.a, I.b); break;
+    case 7: tile2d<T, ACC, 7>(B, S, stage, accumulate, I.a, I.b); break;
+    case 9: tile2d<T, ACC, 9>(B, S, stage, accumulate, I.a, I.b); break;
+    case 11: tile2d<T, ACC, 11>(B, S, stage, accumulate, I.a, I.b); break;
+    case 13: tile2d<T, ACC, 13>(B, S, stage, accumulate, I.a, I.b); break;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_batch_items(const BProb* __restrict__ probs,
+                                                      const BItem* __restrict__ items,
+                                                      int nitems, int stage, int accumulate)
+{
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BItem I = items[it];
+        const BProb& B = probs[I.prob];
+        const int acc = B.plan_accum;
+        if constexpr (sizeof(T) == 8) {
+            run_item<T, kAccF64>(B, I, stage, accumulate);
+        } else {
+            if (acc == kAccF32Kahan) run_item<T, kAccF32Kahan>(B, I, stage, accumulate);
+            else if (acc == kAccF64) run_item<T, kAccF64>(B, I, stage, accumulate);
+            else run_item<T, kAccF32>(B, I, stage, accumulate);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_batch_items(int storage, const BProb* probs, const BItem* items, int nitems,
+                              int stage, int accumulate, int grid, hipStream_t s)
+{
+    if (nitems < 1) return hipSuccess;
+    const unsigned g = (unsigned)(nitems < grid ? nitems : grid);
+    switch (storage) {
+    case kF64:
+        hipLaunchKernelGGL(k_batch_items<double>, dim3(g), dim3(kBlock), 0, s, probs, items, nitems,
+                           stage, accumulate);
+        break;
+    case kF32:
+        hipLaunchKernelGGL(k_batch_items<float>, dim3(g), dim3(kBlock), 0, s, probs, items, nitems,
+                           stage, accumulate);
+        break;
+    case kBF16:
+        hipLaunchKernelGGL(k_batch_items<bf16>, dim3(g), dim3(kBlock), 0, s, probs, items, nitems,
+                           stage, accumulate);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dop

@@ -22,8 +22,10 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     const PlanDev& P = H.meta;
     if (!P.ghost || (P.ndim != 2 && P.ndim != 3)) return;
     const bool f64 = P.storage == kF64 && P.accum == kAccF64;
-    const bool f32 = (P.storage == kF32 || P.storage == kBF16) && P.accum == kAccF32;
+    const bool f32 = (P.storage == kF32 || P.storage == kBF16) &&
+                     (P.accum == kAccF32 || P.accum == kAccF32Kahan);
     if (!f64 && !f32) return;
+    fp->kahan = P.accum == kAccF32Kahan ? 1 : 0;
     // one symmetric window for all axes
     int half = 0;
     for (int a = 0; a < P.ndim; ++a) {
@@ -64,8 +66,6 @@ void build_fast(const HostPlan& H, FastPlan* fp)
         for (size_t i = 0; i < fp->w64[a].size(); ++i) fp->w32[a][i] = (float)fp->w64[a][i];
         A.uniform[a] = X.uniform;
     }
-    A.xbase = (A.lo[0] / fp->VX) * fp->VX;
-    A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, 64 * fp->VX);
     // shell: the domain minus the box, slabs from the slowest axis down
     ShellArgs& S = fp->shell;
     S.nbox = 0;
@@ -95,10 +95,12 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     fp->kind = P.ndim;
 }
 
+// Candidates (autotuner): 2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap;
+// 3-D: chunk depth {64,32,128} x points per lane {widest, half} x XCD remap.
 int fast_variant_count(const FastPlan& fp)
 {
-    if (fp.kind == 2) return 8;   // 4 chunk heights x XCD remap on/off
-    if (fp.kind == 3) return 6;   // 3 chunk depths x XCD remap on/off
+    if (fp.kind == 2) return 16;
+    if (fp.kind == 3) return fp.VX > 1 ? 12 : 6;
     return 1;
 }
 
@@ -107,17 +109,25 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
     if (fp.kind == 0 || v < 0 || v >= fast_variant_count(fp)) return false;
     FastArgs A = fp.args;
     A.xcd_remap = (v & 1) ? 0 : 1;
-    const int c = v >> 1;
     if (fp.kind == 2) {
-        static const int chunks[4] = {32, 16, 64, 128};
-        A.chunk = chunks[c];
+        static const int chunks[4] = {32, 16, 64, 8};
+        A.chunk = chunks[(v >> 1) & 3];
+        A.PF = ((v >> 3) & 1) ? 4 : 2;
+        A.VX = fp.VX;
+    } else {
+        static const int chunks[3] = {64, 32, 128};
+        A.chunk = chunks[(v >> 1) % 3];
+        A.PF = 2;
+        A.VX = (v / 6) ? fp.VX / 2 : fp.VX;
+    }
+    A.xbase = (A.lo[0] / A.VX) * A.VX;
+    A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, 64 * A.VX);
+    if (fp.kind == 2) {
         A.gx = (int)ceil_div(A.nxseg, 4);
         const int64_t nch = ceil_div(A.hi[1] - A.lo[1], A.chunk);
         A.nyblk = 0;
         A.nblocks = (int)(A.gx * nch);
     } else {
-        static const int chunks[3] = {64, 32, 128};
-        A.chunk = chunks[c];
         A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], kWaves3D);
         const int64_t nzc = ceil_div(A.hi[2] - A.lo[2], A.chunk);
         A.gx = 0;

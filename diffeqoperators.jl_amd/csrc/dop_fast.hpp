@@ -27,6 +27,8 @@ struct FastArgs {
     int64_t n[3];
     int64_t s1, s2;        // row / plane strides
     int32_t K;             // window taps on every axis
+    int32_t VX;            // points per lane along x (16-, 8- or 4-byte lane loads)
+    int32_t PF;            // rows / planes prefetched ahead in the register ring
     int32_t lo[3], hi[3];  // interior box [lo, hi) per axis
     int32_t xbase;         // aligned x of the first segment
     int32_t nxseg;         // x segments (64*VX points) per row
@@ -42,7 +44,8 @@ struct FastArgs {
 
 struct FastPlan {
     int kind = 0;          // 0: generic only, 2: 2-D tiled, 3: 3-D tiled
-    int VX = 1;            // points per lane along x
+    int VX = 1;            // widest points-per-lane the row alignment allows
+    int kahan = 0;         // compensated f32 accumulation: batched tiles only
     FastArgs args{};
     ShellArgs shell{};
     int64_t shell_total = 0;

@@ -59,6 +59,35 @@ struct VecIO<float, 4> {
 };
 
 template <>
+struct VecIO<float, 2> {
+    __device__ static void ld(const float* p, float* o)
+    {
+        const float2 v = *reinterpret_cast<const float2*>(p);
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+    __device__ static void st(float* p, const float* v)
+    {
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    }
+};
+
+template <>
+struct VecIO<bf16, 2> {
+    __device__ static void ld(const bf16* p, float* o)
+    {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+        o[0] = __builtin_bit_cast(float, v << 16);
+        o[1] = __builtin_bit_cast(float, v & 0xffff0000u);
+    }
+    __device__ static void st(bf16* p, const float* v)
+    {
+        *reinterpret_cast<uint32_t*>(p) =
+            (uint32_t)f32_to_bf16(v[0]) | ((uint32_t)f32_to_bf16(v[1]) << 16);
+    }
+};
+
+template <>
 struct VecIO<bf16, 4> {
     __device__ static void ld(const bf16* p, float* o)
     {
@@ -114,13 +143,12 @@ __device__ inline int tile_of(int b, int nb, int remap)
 }
 
 // ---------------------------------------------------------------------------------------
-template <typename T, int VX, int K, bool UNI>
+template <typename T, int VX, int K, bool UNI, int PF>
 __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, T* __restrict__ du,
                                                 const T* __restrict__ u, int accumulate)
 {
     using C = typename Store<T>::C;
     constexpr int H = K / 2;
-    constexpr int PF = 2;
     constexpr int R = K + PF;
     constexpr int HA = ((H + VX - 1) / VX) * VX;
     constexpr int NWIN = VX + 2 * HA;
@@ -409,12 +437,15 @@ hipError_t launch_interior(const FastPlan& fp, const FastArgs& A, void* du, cons
 {
     const bool uni = A.uniform[0] && A.uniform[1] && (fp.kind == 2 || A.uniform[2]);
     if (fp.kind == 2) {
-        if (uni)
-            hipLaunchKernelGGL((k_tiled2d<T, VX, K, true>), dim3(A.nblocks), dim3(256), 0, s, A,
-                               (T*)du, (const T*)u, accumulate);
-        else
-            hipLaunchKernelGGL((k_tiled2d<T, VX, K, false>), dim3(A.nblocks), dim3(256), 0, s, A,
-                               (T*)du, (const T*)u, accumulate);
+#define DOP_L2(UNI, PF)                                                                    \
+    hipLaunchKernelGGL((k_tiled2d<T, VX, K, UNI, PF>), dim3(A.nblocks), dim3(256), 0, s, A, \
+                       (T*)du, (const T*)u, accumulate)
+        if (uni) {
+            if (A.PF == 4) DOP_L2(true, 4); else DOP_L2(true, 2);
+        } else {
+            if (A.PF == 4) DOP_L2(false, 4); else DOP_L2(false, 2);
+        }
+#undef DOP_L2
     } else {
         if constexpr (K <= 9) {
             if (uni)
@@ -468,18 +499,20 @@ hipError_t launch_tiled(const DevicePlan& p, void* du, const void* u, int accumu
     case kF64:
         e = launch_shell<double, kAccF64>(p, du, u, accumulate, s);
         if (e != hipSuccess) return e;
-        return p.fast.VX == 2 ? launch_interior_k<double, 2>(p.fast, A, du, u, accumulate, s)
-                              : launch_interior_k<double, 1>(p.fast, A, du, u, accumulate, s);
+        return A.VX == 2 ? launch_interior_k<double, 2>(p.fast, A, du, u, accumulate, s)
+                         : launch_interior_k<double, 1>(p.fast, A, du, u, accumulate, s);
     case kF32:
         e = launch_shell<float, kAccF32>(p, du, u, accumulate, s);
         if (e != hipSuccess) return e;
-        return p.fast.VX == 4 ? launch_interior_k<float, 4>(p.fast, A, du, u, accumulate, s)
-                              : launch_interior_k<float, 1>(p.fast, A, du, u, accumulate, s);
+        if (A.VX == 4) return launch_interior_k<float, 4>(p.fast, A, du, u, accumulate, s);
+        if (A.VX == 2) return launch_interior_k<float, 2>(p.fast, A, du, u, accumulate, s);
+        return launch_interior_k<float, 1>(p.fast, A, du, u, accumulate, s);
     case kBF16:
         e = launch_shell<bf16, kAccF32>(p, du, u, accumulate, s);
         if (e != hipSuccess) return e;
-        return p.fast.VX == 4 ? launch_interior_k<bf16, 4>(p.fast, A, du, u, accumulate, s)
-                              : launch_interior_k<bf16, 1>(p.fast, A, du, u, accumulate, s);
+        if (A.VX == 4) return launch_interior_k<bf16, 4>(p.fast, A, du, u, accumulate, s);
+        if (A.VX == 2) return launch_interior_k<bf16, 2>(p.fast, A, du, u, accumulate, s);
+        return launch_interior_k<bf16, 1>(p.fast, A, du, u, accumulate, s);
     }
     return hipErrorInvalidValue;
 }

@@ -87,6 +87,8 @@ def lib():
     L.dop_plan_autotune.argtypes = [vp, vp, C.POINTER(i)]
     L.dop_batch_create.argtypes = [i, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     L.dop_batch_apply.argtypes = [vp, i, vp]
+    L.dop_batch_autotune.argtypes = [vp, vp, C.POINTER(i)]
+    L.dop_rk4_chunk.argtypes = [vp]
     L.dop_batch_destroy.argtypes = [vp]
     L.dop_batch_destroy.restype = None
     L.dop_rk4_create.argtypes = [i, C.POINTER(vp), _dp, _dp, C.POINTER(vp), vp, C.POINTER(vp)]

@@ -41,6 +41,15 @@ class Batch:
         _abi.check(_abi.lib().dop_batch_apply(self.handle, int(bool(accumulate)),
                                               C.c_void_p(stream)), "dop_batch_apply")
 
+    def autotune(self, stream=None):
+        """Pick the row-chunk height of the 2-D tiles by timing (overwrites the outputs)."""
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        c = C.c_int()
+        _abi.check(_abi.lib().dop_batch_autotune(self.handle, C.c_void_p(stream), C.byref(c)),
+                   "dop_batch_autotune")
+        return c.value
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -72,6 +81,11 @@ class RK4:
             n, P, self._scale.ctypes.data_as(_abi._dp), self._dt.ctypes.data_as(_abi._dp), U,
             C.c_void_p(stream), C.byref(h)), "dop_rk4_create")
         self.handle = h
+
+    @property
+    def chunk(self):
+        """Row-chunk height of the 2-D tiles chosen when the integrator was created."""
+        return _abi.lib().dop_rk4_chunk(self.handle)
 
     def step(self, nsteps=1, stream=None):
         if stream is None:

@@ -164,17 +164,23 @@ typedef struct dop_batch dop_batch;
 int dop_batch_create(int n, const dop_plan* const* plans, void* const* du,
                      const void* const* u, dop_batch** out);
 int dop_batch_apply(const dop_batch* batch, int accumulate, void* stream);
+/* Times the candidate row-chunk heights of the batch's 2-D tiles (on its own buffers: du is
+ * overwritten) and keeps the fastest; *choice receives the chunk height (may be NULL). */
+int dop_batch_autotune(dop_batch* batch, void* stream, int* choice);
 void dop_batch_destroy(dop_batch* batch);
 
 /* ---- explicit time stepping (no counterpart in the reference; its RHS is called from
  * OrdinaryDiffEq, heat_equation.jl:30-33) ----
  * A batch of method-of-lines problems du/dt = scale_i * (sum_t L_t Q u) advanced with
  * classical RK4, every stage of the whole batch one persistent launch, the four stages of
- * a step captured once in a hipGraph and replayed.  Storage f32 or f64. */
+ * a step captured once in a hipGraph and replayed.  Storage f32 or f64.  Creation tunes
+ * the tile shape (DOP_RK4_AUTOTUNE=0 skips it) by timing stage-0 launches, which read u and
+ * write only the internal stage buffers. */
 typedef struct dop_rk4 dop_rk4;
 int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale,
                    const double* dt, void* const* u, void* stream, dop_rk4** out);
 int dop_rk4_step(dop_rk4* rk, int nsteps, void* stream);   /* advances every problem */
+int dop_rk4_chunk(const dop_rk4* rk);   /* row-chunk height picked by the tile autotuner */
 void dop_rk4_destroy(dop_rk4* rk);
 
 /* ---- tile autotuner (no counterpart in the reference) ----

@@ -98,3 +98,25 @@ def test_autotune_candidates_agree():
     out = torch.empty_like(u)
     p.apply(out, u)
     assert torch.equal(out, ref)
+
+
+def plan_with_variant(A, u, v):
+    os.environ["DOP_TILE_VARIANT"] = str(v)
+    try:
+        A2 = D.DiffEqOperatorCombination(A.ops)
+        return D.plans_for(A2, u)[0]
+    finally:
+        os.environ.pop("DOP_TILE_VARIANT", None)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("shape,kh,nvar", [((300, 260), 2, 16), ((40, 72, 100), 3, 12)])
+def test_every_tile_variant_bitwise(shape, kh, nvar, dtype):
+    A, u = build(shape, kh, dtype, seed=5, uniform=False, upwind=(kh == 3))
+    ref = torch.empty_like(u)
+    generic_of(A, u).apply(ref, u)
+    for v in range(nvar):
+        print(f"case: variant {v} {shape} {dtype}", flush=True)
+        out = torch.empty_like(u)
+        plan_with_variant(A, u, v).apply(out, u)
+        assert torch.equal(out, ref), v
