@@ -14,8 +14,9 @@ Arrays are C-ordered tensors whose shape is the Julia size reversed: Julia axis 
 tensor dimension -N.
 """
 from . import _abi
-from .apply import Plan, julia_size, mul_, plans_for
+from .apply import Plan, julia_size, mul_, plan_for_shape, plans_for
 from .batch import Batch, RK4
+from .serve import Server
 from .bcs import (AffineBC, ComposedMultiDimBC, Dirichlet0BC, DirichletBC, GeneralBC,
                   MultiDimBC, MultiDimDirectionalBC, Neumann0BC, NeumannBC, PeriodicBC,
                   RobinBC, compose, decompose)
@@ -30,7 +31,8 @@ __all__ = [
     "RobinBC", "DirichletBC", "Dirichlet0BC", "NeumannBC", "Neumann0BC", "MultiDimBC",
     "MultiDimDirectionalBC", "ComposedMultiDimBC", "AffineBC", "PeriodicBC", "GeneralBC",
     "compose", "decompose", "GhostDerivativeOperator", "DiffEqOperatorCombination",
-    "mul_", "mul", "plans_for", "Plan", "Batch", "RK4", "julia_size",
+    "mul_", "mul", "plans_for", "plan_for_shape", "Plan", "Batch", "RK4", "Server",
+    "julia_size",
 ]
 
 

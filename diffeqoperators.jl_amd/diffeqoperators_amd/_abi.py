@@ -95,6 +95,14 @@ def lib():
     L.dop_rk4_step.argtypes = [vp, i, vp]
     L.dop_rk4_destroy.argtypes = [vp]
     L.dop_rk4_destroy.restype = None
+    L.dop_server_create.argtypes = [C.c_int64, i, i, vp, C.POINTER(vp)]
+    L.dop_server_submit.argtypes = [vp, vp, C.c_double, C.c_double, i, vp, vp, C.c_double,
+                                    C.POINTER(C.c_int64)]
+    L.dop_server_run.argtypes = [vp]
+    L.dop_server_times.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
+    L.dop_server_stats.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.dop_server_destroy.argtypes = [vp]
+    L.dop_server_destroy.restype = None
     if L.dop_version() != ABI_VERSION:
         raise RuntimeError(f"libdop ABI {L.dop_version()} != {ABI_VERSION}")
     _lib = L

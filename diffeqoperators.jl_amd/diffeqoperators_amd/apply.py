@@ -320,3 +320,16 @@ def plans_for(A, u, accum=_abi.ACC_AUTO):
     if terms is None:
         raise TypeError("plans_for needs L*Q or a sum of L*Q")
     return ghost_plans(A, terms, u, accum)
+
+
+def plan_for_shape(A, shape, dtype=torch.float32, device="cuda", accum=_abi.ACC_AUTO):
+    """The single fused plan of L*Q (or a sum of them) for an array of the given tensor
+    shape and dtype, without needing the array itself."""
+    u = torch.empty(1, dtype=dtype, device=device).as_strided(tuple(shape), (0,) * len(shape))
+    terms = _ghost_terms(A)
+    if terms is None:
+        raise TypeError("plan_for_shape needs L*Q or a sum of L*Q")
+    plans = ghost_plans(A, terms, u, accum)
+    if len(plans) != 1:
+        raise ValueError("the operator needs one BC per axis to fuse into one plan")
+    return plans[0]

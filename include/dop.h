@@ -183,6 +183,30 @@ int dop_rk4_step(dop_rk4* rk, int nsteps, void* stream);   /* advances every pro
 int dop_rk4_chunk(const dop_rk4* rk);   /* row-chunk height picked by the tile autotuner */
 void dop_rk4_destroy(dop_rk4* rk);
 
+/* ---- serving (no counterpart in the reference) ----
+ * A continuous-batching method-of-lines service.  Each request is a Q*u plan with its
+ * scale, step dt, number of RK4 steps, an initial field and a result buffer in pinned host
+ * memory, and a release time (seconds after dop_server_run starts).  dop_server_run
+ * replays the releases in time, admits released requests in FIFO order while their state
+ * (u and three RK4 stage buffers, carved from one device arena of `hbm_budget_bytes`) fits
+ * and fewer than `max_active` are resident, advances every resident request by one RK4
+ * step per iteration (each stage one persistent launch over all of them), and copies each
+ * finished state back asynchronously; requests that arrive while others run join at the
+ * next iteration.  Storage f32 or f64 (one type per server). */
+typedef struct dop_server dop_server;
+int dop_server_create(int64_t hbm_budget_bytes, int max_active, int chunk, void* stream,
+                      dop_server** out);
+int dop_server_submit(dop_server* srv, const dop_plan* plan, double scale, double dt, int nsteps,
+                      const void* u0_host, void* result_host, double release_s, int64_t* id);
+int dop_server_run(dop_server* srv);   /* returns when every submitted request completed */
+/* times (s, relative to the run start) of request `id`: release, admission, completion */
+int dop_server_times(const dop_server* srv, int64_t id, double* release, double* admitted,
+                     double* completed);
+/* peak resident requests and peak arena bytes during the last run */
+int dop_server_stats(const dop_server* srv, int* peak_active, int64_t* peak_bytes,
+                     int64_t* iterations);
+void dop_server_destroy(dop_server* srv);
+
 /* ---- tile autotuner (no counterpart in the reference) ----
  * Times the candidate tile shapes of the plan's fused kernel on scratch buffers and keeps
  * the fastest in the plan (and in a process-wide cache keyed by dtype, window and
