@@ -135,7 +135,10 @@ DOP_HD inline double ghost_right(const AxisDev& X, const T* src, int64_t base, i
     return g + X.br;
 }
 
-// FP64 value of a boundary row (always evaluated in double)
+constexpr int kEvalUnroll = 8;
+
+// FP64 value of a boundary row (always evaluated in double).  The ghosts a row touches are
+// formed once, and every tap's load is issued before the first FMA.
 template <typename T>
 DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* src, int r,
                                   int64_t base, int64_t S)
@@ -145,12 +148,32 @@ DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* s
     const int nt = X.bn[b];
     const double* w = X.bw + (int64_t)b * X.KB;
     const int joff = P.ghost ? -1 : 0;
+    const bool gl_used = P.ghost && J0 == 0;
+    const bool gr_used = P.ghost && J0 + nt - 1 == X.n + 1;
+    const double gl = gl_used ? ghost_left<T>(X, src, base, S) : 0.0;
+    const double gr = gr_used ? ghost_right<T>(X, src, base, S) : 0.0;
     double s = 0.0;
+    if (nt <= kEvalUnroll) {
+        double v[kEvalUnroll];
+#pragma unroll
+        for (int k = 0; k < kEvalUnroll; ++k) {
+            if (k < nt) {
+                const int J = J0 + k;
+                if (gl_used && J == 0) v[k] = gl;
+                else if (gr_used && J == X.n + 1) v[k] = gr;
+                else v[k] = (double)Store<T>::ld(src, base + (int64_t)(J + joff) * S);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kEvalUnroll; ++k)
+            if (k < nt) s = fma(w[k], v[k], s);
+        return s;
+    }
     for (int k = 0; k < nt; ++k) {
         const int J = J0 + k;
         double v;
-        if (P.ghost && J == 0) v = ghost_left<T>(X, src, base, S);
-        else if (P.ghost && J == X.n + 1) v = ghost_right<T>(X, src, base, S);
+        if (gl_used && J == 0) v = gl;
+        else if (gr_used && J == X.n + 1) v = gr;
         else v = (double)Store<T>::ld(src, base + (int64_t)(J + joff) * S);
         s = fma(w[k], v, s);
     }
@@ -173,8 +196,24 @@ DOP_HD inline double eval_point(const PlanDev& P, const T* src, const int64_t* i
         if (r >= X.rlo && r < X.rhi) {
             const auto* w = Acc<ACC>::table(X) + (X.uniform ? 0 : (int64_t)r * X.K);
             const int64_t p0 = base + (int64_t)(r + X.dmin + (P.ghost ? 0 : 1)) * S;
-            for (int k = 0; k < X.K; ++k)
-                acc.add(w[k], (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
+            // all taps' loads are issued before the first FMA (K <= kEvalUnroll covers
+            // every centered/upwind window of approximation order <= 8 without a loop)
+            if (X.K <= kEvalUnroll) {
+                typename Acc<ACC>::W wv[kEvalUnroll], xv[kEvalUnroll];
+#pragma unroll
+                for (int k = 0; k < kEvalUnroll; ++k) {
+                    if (k < X.K) {
+                        wv[k] = w[k];
+                        xv[k] = (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kEvalUnroll; ++k)
+                    if (k < X.K) acc.add(wv[k], xv[k]);
+            } else {
+                for (int k = 0; k < X.K; ++k)
+                    acc.add(w[k], (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
+            }
         } else {
             acc.add_sum(boundary_row<T>(P, X, src, r, base, S));
         }

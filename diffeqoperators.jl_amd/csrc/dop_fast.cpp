@@ -95,12 +95,17 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     fp->kind = P.ndim;
 }
 
-// Candidates (autotuner): 2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap;
-// 3-D: chunk depth {64,32,128} x points per lane {widest, half} x XCD remap.
+// Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap x
+// points per lane {widest 16-byte, twice that}; 3-D: chunk depth {64,32} x prefetch {2,4} x
+// XCD remap x points per lane {widest, half}.
+namespace {
+bool wide_ok(const FastPlan& fp) { return fp.VX > 1 && fp.args.n[0] % (2 * fp.VX) == 0; }
+}  // namespace
+
 int fast_variant_count(const FastPlan& fp)
 {
-    if (fp.kind == 2) return 16;
-    if (fp.kind == 3) return fp.VX > 1 ? 12 : 6;
+    if (fp.kind == 2) return wide_ok(fp) ? 32 : 16;
+    if (fp.kind == 3) return fp.VX > 1 ? 16 : 8;
     return 1;
 }
 
@@ -113,12 +118,12 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         static const int chunks[4] = {32, 16, 64, 8};
         A.chunk = chunks[(v >> 1) & 3];
         A.PF = ((v >> 3) & 1) ? 4 : 2;
-        A.VX = fp.VX;
+        A.VX = ((v >> 4) & 1) ? 2 * fp.VX : fp.VX;
     } else {
-        static const int chunks[3] = {64, 32, 128};
-        A.chunk = chunks[(v >> 1) % 3];
-        A.PF = 2;
-        A.VX = (v / 6) ? fp.VX / 2 : fp.VX;
+        static const int chunks[2] = {64, 32};
+        A.chunk = chunks[(v >> 1) & 1];
+        A.PF = ((v >> 2) & 1) ? 4 : 2;
+        A.VX = ((v >> 3) & 1) ? fp.VX / 2 : fp.VX;
     }
     A.xbase = (A.lo[0] / A.VX) * A.VX;
     A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, 64 * A.VX);

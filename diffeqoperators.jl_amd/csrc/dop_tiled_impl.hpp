@@ -1,0 +1,563 @@
+// dop_tiled_impl.hpp -- tiled interior kernels for 2-D / 3-D Q*u plans, instantiated per
+// storage type by dop_tiled_f64.hip / dop_tiled_f32.hip / dop_tiled_bf16.hip.
+//
+// 2-D: a wave owns a 64*VX-point x segment and marches a chunk of rows.  The K rows of the
+//      y stencil live in a register ring of K+PF rows (PF rows prefetched ahead), so every
+//      row of u is loaded once per chunk; the x window of the centre row comes from lane
+//      loads that hit L1 (the row was loaded by the same wave PF+H steps earlier).
+// 3-D: a block of 8 waves owns 8 consecutive y rows of one x segment and marches a chunk
+//      of z planes with the same register ring along z.  Each step stages the centre plane
+//      (own rows + 2H halo rows) in LDS, double-buffered, one barrier per step; y
+//      neighbours are read from LDS, x neighbours from L1.
+// Both sum, per point, the x window, then y, then z, each in tap order: the same FMA
+// sequence as the generic path, so the two agree bitwise on finite inputs.
+//
+// The points outside the interior box (the shell) are evaluated by the first blocks of the
+// same launch with the generic per-point path, so they overlap the tiles.
+//
+// Blocks are dispatched round-robin over the 8 XCDs; with xcd_remap the logical tile of a
+// block is chosen so that each XCD walks a contiguous band of tiles (halo rows of
+// neighbouring tiles then hit the same L2).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "dop_eval.hpp"
+#include "dop_kernels.hpp"
+
+namespace dop {
+namespace tiled {
+
+
+// ---------------------------------------------------------------------------------------
+// VX consecutive elements per lane
+template <typename T, int VX>
+struct VecIO;
+
+template <>
+struct VecIO<double, 2> {
+    __device__ static void ld(const double* p, double* o)
+    {
+        const double2 v = *reinterpret_cast<const double2*>(p);
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+    __device__ static void st(double* p, const double* v)
+    {
+        *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    }
+};
+
+template <>
+struct VecIO<float, 4> {
+    __device__ static void ld(const float* p, float* o)
+    {
+        const float4 v = *reinterpret_cast<const float4*>(p);
+        o[0] = v.x;
+        o[1] = v.y;
+        o[2] = v.z;
+        o[3] = v.w;
+    }
+    __device__ static void st(float* p, const float* v)
+    {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+
+template <>
+struct VecIO<float, 2> {
+    __device__ static void ld(const float* p, float* o)
+    {
+        const float2 v = *reinterpret_cast<const float2*>(p);
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+    __device__ static void st(float* p, const float* v)
+    {
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    }
+};
+
+template <>
+struct VecIO<bf16, 2> {
+    __device__ static void ld(const bf16* p, float* o)
+    {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+        o[0] = __builtin_bit_cast(float, v << 16);
+        o[1] = __builtin_bit_cast(float, v & 0xffff0000u);
+    }
+    __device__ static void st(bf16* p, const float* v)
+    {
+        *reinterpret_cast<uint32_t*>(p) =
+            (uint32_t)f32_to_bf16(v[0]) | ((uint32_t)f32_to_bf16(v[1]) << 16);
+    }
+};
+
+template <>
+struct VecIO<bf16, 4> {
+    __device__ static void ld(const bf16* p, float* o)
+    {
+        const uint2 v = *reinterpret_cast<const uint2*>(p);
+        o[0] = __builtin_bit_cast(float, v.x << 16);
+        o[1] = __builtin_bit_cast(float, v.x & 0xffff0000u);
+        o[2] = __builtin_bit_cast(float, v.y << 16);
+        o[3] = __builtin_bit_cast(float, v.y & 0xffff0000u);
+    }
+    __device__ static void st(bf16* p, const float* v)
+    {
+        uint2 w;
+        w.x = (uint32_t)f32_to_bf16(v[0]) | ((uint32_t)f32_to_bf16(v[1]) << 16);
+        w.y = (uint32_t)f32_to_bf16(v[2]) | ((uint32_t)f32_to_bf16(v[3]) << 16);
+        *reinterpret_cast<uint2*>(p) = w;
+    }
+};
+
+// two 16-byte loads per lane
+template <>
+struct VecIO<double, 4> {
+    __device__ static void ld(const double* p, double* o)
+    {
+        VecIO<double, 2>::ld(p, o);
+        VecIO<double, 2>::ld(p + 2, o + 2);
+    }
+    __device__ static void st(double* p, const double* v)
+    {
+        VecIO<double, 2>::st(p, v);
+        VecIO<double, 2>::st(p + 2, v + 2);
+    }
+};
+
+template <>
+struct VecIO<float, 8> {
+    __device__ static void ld(const float* p, float* o)
+    {
+        VecIO<float, 4>::ld(p, o);
+        VecIO<float, 4>::ld(p + 4, o + 4);
+    }
+    __device__ static void st(float* p, const float* v)
+    {
+        VecIO<float, 4>::st(p, v);
+        VecIO<float, 4>::st(p + 4, v + 4);
+    }
+};
+
+template <>
+struct VecIO<bf16, 8> {
+    __device__ static void ld(const bf16* p, float* o)
+    {
+        const uint4 v = *reinterpret_cast<const uint4*>(p);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            o[2 * i] = __builtin_bit_cast(float, w[i] << 16);
+            o[2 * i + 1] = __builtin_bit_cast(float, w[i] & 0xffff0000u);
+        }
+    }
+    __device__ static void st(bf16* p, const float* v)
+    {
+        uint32_t w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            w[i] = (uint32_t)f32_to_bf16(v[2 * i]) | ((uint32_t)f32_to_bf16(v[2 * i + 1]) << 16);
+        *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+};
+
+template <typename T>
+struct VecIO1 {
+    using C = typename Store<T>::C;
+    __device__ static void ld(const T* p, C* o) { o[0] = (C)Store<T>::ld(p, 0); }
+    __device__ static void st(T* p, const C* v) { Store<T>::st(p, 0, (double)v[0]); }
+};
+template <>
+struct VecIO<double, 1> : VecIO1<double> {};
+template <>
+struct VecIO<float, 1> : VecIO1<float> {};
+template <>
+struct VecIO<bf16, 1> : VecIO1<bf16> {};
+
+template <typename C>
+__device__ inline C fmac(C a, C b, C c);
+template <>
+__device__ inline double fmac<double>(double a, double b, double c) { return fma(a, b, c); }
+template <>
+__device__ inline float fmac<float>(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename C>
+__device__ inline const C* wtab(const FastArgs& A, int a);
+template <>
+__device__ inline const double* wtab<double>(const FastArgs& A, int a) { return A.w64[a]; }
+template <>
+__device__ inline const float* wtab<float>(const FastArgs& A, int a) { return A.w32[a]; }
+
+// logical tile of this block: each XCD (blocks b, b+8, b+16, ...) gets a contiguous band
+__device__ inline int tile_of(int b, int nb, int remap)
+{
+    if (!remap) return b;
+    const int per = nb >> 3;
+    if (b >= per * 8) return b;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <typename T>
+constexpr int acc_of() { return std::is_same<T, double>::value ? kAccF64 : kAccF32; }
+
+// generic evaluation of shell points [first, first + count)
+template <typename T, int ACC>
+__device__ inline void shell_points(const PlanDev& P, const ShellArgs& S, T* __restrict__ du,
+                                    const T* __restrict__ src, int accumulate, int64_t first,
+                                    int64_t count)
+{
+    const int64_t total = S.start[S.nbox];
+    const int64_t end = min(first + count, total);
+    for (int64_t p = first + threadIdx.x; p < end; p += blockDim.x) {
+        int b = 0;
+        while (b + 1 < S.nbox && p >= S.start[b + 1]) ++b;
+        int64_t l = p - S.start[b];
+        const int64_t e0 = S.ext[b][0], e1 = S.ext[b][1];
+        int64_t idx[3];
+        idx[0] = S.org[b][0] + l % e0;
+        l /= e0;
+        idx[1] = S.org[b][1] + l % e1;
+        idx[2] = S.org[b][2] + l / e1;
+        const int64_t lin = idx[0] + idx[1] * P.sdu[1] + idx[2] * P.sdu[2];
+        double v = eval_point<T, ACC>(P, src, idx);
+        if (accumulate) v += (double)Store<T>::ld(du, lin);
+        Store<T>::st(du, lin, v);
+    }
+}
+
+constexpr int kShellPtsPerThread = 4;
+
+// ---------------------------------------------------------------------------------------
+template <typename T, int VX, int K, bool UNI, int PF>
+__global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev* __restrict__ Pp,
+                                                const ShellArgs SH, int nshell,
+                                                T* __restrict__ du, const T* __restrict__ u,
+                                                int accumulate)
+{
+    using C = typename Store<T>::C;
+    constexpr int H = K / 2;
+    constexpr int R = K + PF;
+    constexpr int HA = ((H + VX - 1) / VX) * VX;
+    constexpr int NWIN = VX + 2 * HA;
+    if ((int)blockIdx.x < nshell) {
+        const int64_t per = 256 * kShellPtsPerThread;
+        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b = tile_of((int)blockIdx.x - nshell, A.nblocks, A.xcd_remap);
+    const int xg = b % A.gx;
+    const int ch = b / A.gx;
+    const int xseg = xg * 4 + wave;
+    if (xseg >= A.nxseg) return;
+    const int y0 = A.lo[1] + ch * A.chunk;
+    const int y1 = min(y0 + A.chunk, A.hi[1]);
+    const int64_t n0 = A.n[0];
+    const int n1 = (int)A.n[1];
+    const int64_t s1 = A.s1;
+    const int64_t xs = (int64_t)A.xbase + (int64_t)xseg * (64 * VX) + lane * VX;
+    const bool rowok = xs + VX <= n0;
+    const bool full = xs >= A.lo[0] && xs + VX <= A.hi[0];
+
+    const C* wxt = wtab<C>(A, 0);
+    const C* wyt = wtab<C>(A, 1);
+    C wx[VX][K];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+        int64_t xr = 0;
+        if (!UNI) xr = min(max(xs + v, (int64_t)0), n0 - 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) wx[v][k] = wxt[xr * K + k];
+    }
+    C wyu[K];
+    if (UNI) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) wyu[k] = wyt[k];
+    }
+
+    C q[R][VX];
+    auto load_row = [&](int row, C* dst) {
+        row = min(max(row, 0), n1 - 1);
+        const T* p = u + (int64_t)row * s1 + xs;
+        if (rowok) {
+            VecIO<T, VX>::ld(p, dst);
+        } else {
+#pragma unroll
+            for (int v = 0; v < VX; ++v) dst[v] = (xs + v < n0) ? (C)Store<T>::ld(p, v) : C(0);
+        }
+    };
+#pragma unroll
+    for (int s = 0; s < R - 1; ++s) load_row(y0 - H + s, q[s]);
+
+    for (int yb = y0; yb < y1; yb += R) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int y = yb + j;
+            if (y >= y1) break;
+            load_row(y + H + PF, q[(j + K - 1 + PF) % R]);
+            C xw[NWIN];
+#pragma unroll
+            for (int v = 0; v < VX; ++v) xw[HA + v] = q[(j + H) % R][v];
+            const T* prow = u + (int64_t)y * s1;
+            if (full) {
+#pragma unroll
+                for (int c = 0; c < HA / VX; ++c) {
+                    VecIO<T, VX>::ld(prow + xs - HA + c * VX, &xw[c * VX]);
+                    VecIO<T, VX>::ld(prow + xs + VX + c * VX, &xw[HA + VX + c * VX]);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < HA; ++e) {
+                    const int64_t xl = xs - HA + e, xr = xs + VX + e;
+                    xw[e] = (xl >= 0 && xl < n0) ? (C)Store<T>::ld(prow, xl) : C(0);
+                    xw[HA + VX + e] = (xr >= 0 && xr < n0) ? (C)Store<T>::ld(prow, xr) : C(0);
+                }
+            }
+            C wy[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) wy[k] = UNI ? wyu[k] : wyt[(int64_t)y * K + k];
+            C out[VX];
+#pragma unroll
+            for (int v = 0; v < VX; ++v) {
+                C s = C(0);
+#pragma unroll
+                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[HA - H + v + k], s);
+#pragma unroll
+                for (int k = 0; k < K; ++k) s = fmac<C>(wy[k], q[(j + k) % R][v], s);
+                out[v] = s;
+            }
+            T* po = du + (int64_t)y * s1 + xs;
+            if (full && !accumulate) {
+                VecIO<T, VX>::st(po, out);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VX; ++v) {
+                    const int64_t x = xs + v;
+                    if (x >= A.lo[0] && x < A.hi[0]) {
+                        double r = (double)out[v];
+                        if (accumulate) r += (double)Store<T>::ld(po, v);
+                        Store<T>::st(po, v, r);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+constexpr int kW3 = 8;   // waves (= y rows) per 3-D block
+
+template <typename T, int VX, int K, bool UNI, int PF>
+__global__ __launch_bounds__(64 * kW3) void k_tiled3d(const FastArgs A, const PlanDev* __restrict__ Pp,
+                                                     const ShellArgs SH, int nshell,
+                                                     T* __restrict__ du, const T* __restrict__ u,
+                                                     int accumulate)
+{
+    using C = typename Store<T>::C;
+    constexpr int H = K / 2;
+    constexpr int R = K + PF;
+    constexpr int HA = ((H + VX - 1) / VX) * VX;
+    constexpr int NWIN = VX + 2 * HA;
+    constexpr int ROWW = 64 * VX;
+    constexpr int LROWS = kW3 + 2 * H;
+    static_assert(2 * H <= kW3, "halo rows are staged by one wave each");
+    __shared__ C lds[2][LROWS][ROWW];
+    if ((int)blockIdx.x < nshell) {   // whole block: no barrier is reached
+        const int64_t per = 64 * kW3 * kShellPtsPerThread;
+        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
+        return;
+    }
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b = tile_of((int)blockIdx.x - nshell, A.nblocks, A.xcd_remap);
+    const int xseg = b % A.nxseg;
+    const int t = b / A.nxseg;
+    const int yblk = t % A.nyblk;
+    const int zch = t / A.nyblk;
+    const int yb0 = A.lo[1] + yblk * kW3;
+    const int y = yb0 + wave;   // this wave's output row
+    const bool yok = y < A.hi[1];
+    const int z0 = A.lo[2] + zch * A.chunk;
+    const int z1 = min(z0 + A.chunk, A.hi[2]);
+    const int64_t n0 = A.n[0];
+    const int n1 = (int)A.n[1], n2 = (int)A.n[2];
+    const int64_t s1 = A.s1, s2 = A.s2;
+    const int64_t xs = (int64_t)A.xbase + (int64_t)xseg * ROWW + lane * VX;
+    const bool rowok = xs + VX <= n0;
+    const bool full = xs >= A.lo[0] && xs + VX <= A.hi[0];
+    const int yc = min(max(y, 0), n1 - 1);
+    // halo row staged by waves 0 .. 2H-1
+    const bool has_halo = wave < 2 * H;
+    const int yh = wave < H ? yb0 - H + wave : yb0 + kW3 + (wave - H);
+    const int yhc = min(max(yh, 0), n1 - 1);
+    const int hrow = wave < H ? wave : kW3 + wave;
+
+    const C* wxt = wtab<C>(A, 0);
+    const C* wyt = wtab<C>(A, 1);
+    const C* wzt = wtab<C>(A, 2);
+    C wx[VX][K];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+        int64_t xr = 0;
+        if (!UNI) xr = min(max(xs + v, (int64_t)0), n0 - 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) wx[v][k] = wxt[xr * K + k];
+    }
+    C wy[K];
+    {
+        const int64_t yr = UNI ? 0 : yc;
+#pragma unroll
+        for (int k = 0; k < K; ++k) wy[k] = wyt[yr * K + k];
+    }
+    C wzu[K];
+    if (UNI) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) wzu[k] = wzt[k];
+    }
+
+    auto load_at = [&](int yy, int zz, C* dst) {
+        zz = min(max(zz, 0), n2 - 1);
+        const T* p = u + (int64_t)zz * s2 + (int64_t)yy * s1 + xs;
+        if (rowok) {
+            VecIO<T, VX>::ld(p, dst);
+        } else {
+#pragma unroll
+            for (int v = 0; v < VX; ++v) dst[v] = (xs + v < n0) ? (C)Store<T>::ld(p, v) : C(0);
+        }
+    };
+
+    C q[R][VX];
+#pragma unroll
+    for (int s = 0; s < R - 1; ++s) load_at(yc, z0 - H + s, q[s]);
+
+    int buf = 0;
+    for (int zb = z0; zb < z1; zb += R) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int z = zb + j;
+            if (z >= z1) break;   // block-uniform
+            load_at(yc, z + H + PF, q[(j + K - 1 + PF) % R]);
+            C hv[VX];
+            if (has_halo) load_at(yhc, z, hv);
+#pragma unroll
+            for (int v = 0; v < VX; ++v) lds[buf][H + wave][lane * VX + v] = q[(j + H) % R][v];
+            if (has_halo) {
+#pragma unroll
+                for (int v = 0; v < VX; ++v) lds[buf][hrow][lane * VX + v] = hv[v];
+            }
+            __syncthreads();
+            C xw[NWIN];
+#pragma unroll
+            for (int v = 0; v < VX; ++v) xw[HA + v] = q[(j + H) % R][v];
+            const T* prow = u + (int64_t)z * s2 + (int64_t)yc * s1;
+            if (full) {
+#pragma unroll
+                for (int c = 0; c < HA / VX; ++c) {
+                    VecIO<T, VX>::ld(prow + xs - HA + c * VX, &xw[c * VX]);
+                    VecIO<T, VX>::ld(prow + xs + VX + c * VX, &xw[HA + VX + c * VX]);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < HA; ++e) {
+                    const int64_t xl = xs - HA + e, xr = xs + VX + e;
+                    xw[e] = (xl >= 0 && xl < n0) ? (C)Store<T>::ld(prow, xl) : C(0);
+                    xw[HA + VX + e] = (xr >= 0 && xr < n0) ? (C)Store<T>::ld(prow, xr) : C(0);
+                }
+            }
+            C wz[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) wz[k] = UNI ? wzu[k] : wzt[(int64_t)z * K + k];
+            C out[VX];
+#pragma unroll
+            for (int v = 0; v < VX; ++v) {
+                C s = C(0);
+#pragma unroll
+                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[HA - H + v + k], s);
+#pragma unroll
+                for (int k = 0; k < K; ++k) s = fmac<C>(wy[k], lds[buf][wave + k][lane * VX + v], s);
+#pragma unroll
+                for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[(j + k) % R][v], s);
+                out[v] = s;
+            }
+            if (yok) {
+                T* po = du + (int64_t)z * s2 + (int64_t)y * s1 + xs;
+                if (full && !accumulate) {
+                    VecIO<T, VX>::st(po, out);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VX; ++v) {
+                        const int64_t x = xs + v;
+                        if (x >= A.lo[0] && x < A.hi[0]) {
+                            double r = (double)out[v];
+                            if (accumulate) r += (double)Store<T>::ld(po, v);
+                            Store<T>::st(po, v, r);
+                        }
+                    }
+                }
+            }
+            buf ^= 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+template <typename T, int VX, int K>
+hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
+                           int accumulate, hipStream_t s)
+{
+    const FastPlan& fp = p.fast;
+    const bool uni = A.uniform[0] && A.uniform[1] && (fp.kind == 2 || A.uniform[2]);
+    const int threads = fp.kind == 2 ? 256 : 64 * kW3;
+    const int64_t per = (int64_t)threads * kShellPtsPerThread;
+    const int nshell = (int)((fp.shell_total + per - 1) / per);
+    const dim3 grid((unsigned)(A.nblocks + nshell));
+    if (fp.kind == 2) {
+#define DOP_L2(UNI, PF)                                                                      \
+    hipLaunchKernelGGL((k_tiled2d<T, VX, K, UNI, PF>), grid, dim3(256), 0, s, A, p.dev,        \
+                       fp.shell, nshell, (T*)du, (const T*)u, accumulate)
+        if (uni) {
+            if (A.PF == 4) DOP_L2(true, 4); else DOP_L2(true, 2);
+        } else {
+            if (A.PF == 4) DOP_L2(false, 4); else DOP_L2(false, 2);
+        }
+#undef DOP_L2
+    } else {
+        if constexpr (K <= 9 && VX <= 4) {
+#define DOP_L3(UNI, PF)                                                                      \
+    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, PF>), grid, dim3(64 * kW3), 0, s, A, p.dev,   \
+                       fp.shell, nshell, (T*)du, (const T*)u, accumulate)
+            if (uni) {
+                if (A.PF == 4) DOP_L3(true, 4); else DOP_L3(true, 2);
+            } else {
+                if (A.PF == 4) DOP_L3(false, 4); else DOP_L3(false, 2);
+            }
+#undef DOP_L3
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int VX>
+hipError_t launch_interior_k(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
+                             int accumulate, hipStream_t s)
+{
+    switch (A.K) {
+    case 3: return launch_interior<T, VX, 3>(p, A, du, u, accumulate, s);
+    case 5: return launch_interior<T, VX, 5>(p, A, du, u, accumulate, s);
+    case 7: return launch_interior<T, VX, 7>(p, A, du, u, accumulate, s);
+    case 9: return launch_interior<T, VX, 9>(p, A, du, u, accumulate, s);
+    case 11: return launch_interior<T, VX, 11>(p, A, du, u, accumulate, s);
+    case 13: return launch_interior<T, VX, 13>(p, A, du, u, accumulate, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace tiled
+}  // namespace dop
