@@ -163,6 +163,170 @@ def bench_nonuniform3d(steps, warmup, n=512):
 
 
 # ------------------------------------------------------------------------------------------
+def batch_problem(i, dtype):
+    """Problem i of the 1024-problem mix: 1-D or 2-D, N log-uniform, d in 1..4,
+    a in {2,4,6,8}, Dirichlet / Neumann / Robin; RHS sign (-1)^(d/2+1) (even d), -1 (odd)."""
+    import numpy as np
+    import torch
+    import diffeqoperators_amd as D
+    rng = np.random.default_rng(1000 + i)
+    ndim = int(rng.integers(1, 3))
+    d = int(rng.integers(1, 5))
+    a = int(rng.choice([2, 4, 6, 8]))
+    hi = 65536 if ndim == 1 else 2048
+    n = int(np.exp(rng.uniform(np.log(64), np.log(hi))))
+    h = 1.0 / (n + 1)
+    g = float(rng.uniform(-1, 1))
+    kind = int(rng.integers(0, 3))
+    atom = (D.DirichletBC(g, -g) if kind == 0 else
+            D.NeumannBC((g, 0.5 * g), h, 1) if kind == 1 else
+            D.RobinBC((1.0, 0.5, g), (1.0, 0.5, -g), h, 1))
+    if ndim == 1:
+        ops = [D.CenteredDifference(d, a, h, n)]
+        A = ops[0] * atom
+        shape = (n,)
+    else:
+        ops = [D.CenteredDifference[1](d, a, h, n), D.CenteredDifference[2](d, a, h, n)]
+        A = (ops[0] + ops[1]) * D.compose(*D.MultiDimBC(atom, (n, n)))
+        shape = (n, n)
+    sign = -1.0 if d % 2 == 1 else float((-1) ** (d // 2 + 1))
+    wsum = sum(float(np.abs(L.stencil_coefs).sum()) for L in ops)
+    u = torch.from_numpy(rng.uniform(-1, 1, shape)).to("cuda").to(dtype)
+    return A, u, sign, 0.5 / wsum
+
+
+def bench_batched(steps, nprob=1024, plain_sample=48, plain_steps=4):
+    """1024 independent problems, graph-captured RK4, one persistent launch per stage."""
+    import torch
+    import diffeqoperators_amd as D
+    probs = [batch_problem(i, torch.float32) for i in range(nprob)]
+    pts = sum(p[1].numel() for p in probs)
+    t0 = time.perf_counter()
+    rk = D.RK4([p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs],
+               [p[3] for p in probs])
+    t_create = time.perf_counter() - t0
+    rk.step(2)
+    _sync()
+    wall, ev = time_applies(lambda: rk.step(1), steps, 1)
+    step_s = ev / steps
+    # plain per-call form on a sample of the same problems: per stage one dop_apply and
+    # torch combinations, one problem after the other
+    sample = probs[:plain_sample]
+    spts = sum(p[1].numel() for p in sample)
+
+    def plain_step(A, u, sign, dt, bufs):
+        plan = D.plans_for(A, u)[0]
+        k, x, acc = bufs
+        plan.apply(k, u)
+        k.mul_(sign)
+        torch.add(u, k, alpha=dt / 6, out=acc)
+        torch.add(u, k, alpha=dt / 2, out=x)
+        for c, w in ((dt / 2, dt / 3), (dt, dt / 3)):
+            plan.apply(k, x)
+            k.mul_(sign)
+            acc.add_(k, alpha=w)
+            torch.add(u, k, alpha=c, out=x)
+        plan.apply(k, x)
+        k.mul_(sign)
+        torch.add(acc, k, alpha=dt / 6, out=u)
+
+    bufs = [tuple(torch.empty_like(p[1]) for _ in range(3)) for p in sample]
+    for p, b in zip(sample, bufs):
+        plain_step(p[0], p[1], p[2], p[3], b)
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for p, b in zip(sample, bufs):
+            plain_step(p[0], p[1], p[2], p[3], b)
+    _sync()
+    plain_s = (time.perf_counter() - t0) / plain_steps
+    batched_sample_s = step_s * spts / pts   # batched cost of the same points
+    return dict(
+        problems=nprob, points=pts, rk4_step_ms=round(step_s * 1e3, 4), tile_chunk=rk.chunk,
+        rhs_gpts_s=round(4 * pts / step_s / 1e9, 2),
+        problem_steps_per_s=round(nprob / step_s, 1),
+        solves_per_s_100_steps=round(nprob / step_s / 100, 2),
+        plain_sample=dict(problems=plain_sample, points=spts, step_ms=round(plain_s * 1e3, 3),
+                          problem_steps_per_s=round(plain_sample / plain_s, 1)),
+        speedup_vs_plain_same_points=round(plain_s / batched_sample_s, 2),
+        create_s=round(t_create, 2))
+
+
+def served_requests(nreq, lam, seed=5):
+    import numpy as np
+    import torch
+    import diffeqoperators_amd as D
+    rng = np.random.default_rng(seed)
+    rel = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    reqs = []
+    for i in range(nreq):
+        r = np.random.default_rng(5000 + i)
+        n = int(np.exp(r.uniform(np.log(64), np.log(2048))))
+        h = 1.0 / (n + 1)
+        vx, vy = r.standard_normal(n), r.standard_normal(n)
+        ops = [D.CenteredDifference[1](2, 4, h, n), D.CenteredDifference[2](2, 4, h, n),
+               D.UpwindDifference[1](1, 3, h, n, -vx), D.UpwindDifference[2](1, 3, h, n, -vy)]
+        A = (ops[0] + ops[1] + ops[2] + ops[3]) * D.compose(*D.Dirichlet0BC(float, (n, n)))
+        wsum = sum(float(np.abs(L.stencil_coefs).sum()) for L in ops[:2]) + \
+            float(np.abs(np.r_[vx, vy]).max()) * sum(float(np.abs(L.stencil_coefs).sum()) for L in ops[2:])
+        u0 = torch.from_numpy(r.uniform(-1, 1, (n, n)).astype(np.float32)).pin_memory()
+        reqs.append((A, n, u0, 0.4 / wsum, float(rel[i])))
+    return reqs
+
+
+def bench_served(nreq=200, lam=2000.0, nsteps=100, budget=8 << 30):
+    """Poisson-arrival method-of-lines solves: continuous batching vs the plain per-call
+    form (one request at a time, per stage one dop_apply and torch combinations)."""
+    import numpy as np
+    import torch
+    import diffeqoperators_amd as D
+    reqs = served_requests(nreq, lam)
+    plans = [D.plan_for_shape(A, (n, n), torch.float32) for A, n, _, _, _ in reqs]
+    srv = D.Server(budget, max_active=4096, chunk=16)
+    for (A, n, u0, dt, rel), plan in zip(reqs, plans):
+        srv.submit(plan, u0, 1.0, dt, nsteps, rel)
+    srv.run()
+    t = srv.times()
+    lat = t[:, 2] - t[:, 0]
+    makespan = t[:, 2].max() - t[:, 0].min()
+    served = dict(solves_per_s=round(nreq / makespan, 1), p50_ms=round(np.percentile(lat, 50) * 1e3, 2),
+                  p99_ms=round(np.percentile(lat, 99) * 1e3, 2), makespan_s=round(makespan, 3),
+                  **srv.stats())
+    # plain: FIFO, one request at a time
+    lat_p, t0 = [], None
+    res_plain = []
+    start = time.perf_counter()
+    for (A, n, u0, dt, rel), plan in zip(reqs, plans):
+        now = time.perf_counter() - start
+        if now < rel:
+            time.sleep(rel - now)
+        u = u0.to("cuda", non_blocking=True)
+        k, x, acc = torch.empty_like(u), torch.empty_like(u), torch.empty_like(u)
+        for _ in range(nsteps):
+            plan.apply(k, u)
+            torch.add(u, k, alpha=dt / 6, out=acc)
+            torch.add(u, k, alpha=dt / 2, out=x)
+            for c, w in ((dt / 2, dt / 3), (dt, dt / 3)):
+                plan.apply(k, x)
+                acc.add_(k, alpha=w)
+                torch.add(u, k, alpha=c, out=x)
+            plan.apply(k, x)
+            torch.add(acc, k, alpha=dt / 6, out=u)
+        res_plain.append(u.cpu())
+        lat_p.append(time.perf_counter() - start - rel)
+    mk_p = time.perf_counter() - start - reqs[0][4]
+    lat_p = np.array(lat_p)
+    plain = dict(solves_per_s=round(nreq / mk_p, 1), p50_ms=round(np.percentile(lat_p, 50) * 1e3, 2),
+                 p99_ms=round(np.percentile(lat_p, 99) * 1e3, 2), makespan_s=round(mk_p, 3))
+    # equal accuracy: the two forms integrate the same RK4 (FP32 storage)
+    diff = max(float((srv.results[i] - res_plain[i]).abs().max() / res_plain[i].abs().max())
+               for i in range(nreq))
+    return dict(requests=nreq, arrival_rate_per_s=lam, rk4_steps=nsteps, served=served, plain=plain,
+                served_over_plain_solves=round(served["solves_per_s"] / plain["solves_per_s"], 2),
+                max_rel_diff_served_vs_plain=diff)
+
+
+# ------------------------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,9 +334,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="laplacian2d",
                     choices=["laplacian2d", "nonuniform3d"])
-    ap.add_argument("--also", action="store_true", help="add the other workloads under 'also'")
+    ap.add_argument("--no-also", action="store_true",
+                    help="skip the other workloads (3-D non-uniform, batched RK4, served)")
+    ap.add_argument("--requests", type=int, default=200, help="served workload: requests")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-reference", action="store_true",
+                    help="also run a plain 256 MiB device copy (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this build has no "
@@ -183,6 +351,12 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
 
+    if args.copy_reference:
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
     if args.config == "laplacian2d":
         r = bench_laplacian2d(args.steps, args.warmup)
         line = {
@@ -208,8 +382,12 @@ def main():
                          "unit": "GB/s", "frac": f32["frac"], "traffic": None},
             "variants": r3,
         }
-    if args.also and args.config == "laplacian2d":
+    if not args.no_also and args.config == "laplacian2d":
         line["also"] = {"nonuniform3d_512": bench_nonuniform3d(max(20, args.steps // 4), 3)}
+        torch.cuda.empty_cache()
+        line["also"]["batched_rk4_1024"] = bench_batched(max(10, args.steps // 10))
+        torch.cuda.empty_cache()
+        line["also"]["served_advdiff"] = bench_served(args.requests)
     if not args.no_cpu and args.config == "laplacian2d":
         line["cpu_baseline"] = cpu_baseline_laplacian2d(args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -10,7 +10,6 @@ namespace {
 
 constexpr int kMaxK2D = 13;
 constexpr int kMaxK3D = 9;   // 3-D stages 2*H halo rows through 8 waves: H <= 4
-constexpr int kWaves3D = 8;
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -96,8 +95,8 @@ void build_fast(const HostPlan& H, FastPlan* fp)
 }
 
 // Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap x
-// points per lane {widest 16-byte, twice that}; 3-D: chunk depth {64,32} x prefetch {2,4} x
-// XCD remap x points per lane {widest, half}.
+// points per lane {widest 16-byte, twice that}; 3-D: chunk depth {64,32} x waves per block
+// {8,16} x XCD remap x points per lane {widest, half}.
 namespace {
 bool wide_ok(const FastPlan& fp) { return fp.VX > 1 && fp.args.n[0] % (2 * fp.VX) == 0; }
 }  // namespace
@@ -119,10 +118,12 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         A.chunk = chunks[(v >> 1) & 3];
         A.PF = ((v >> 3) & 1) ? 4 : 2;
         A.VX = ((v >> 4) & 1) ? 2 * fp.VX : fp.VX;
+        A.W = 4;
     } else {
         static const int chunks[2] = {64, 32};
         A.chunk = chunks[(v >> 1) & 1];
-        A.PF = ((v >> 2) & 1) ? 4 : 2;
+        A.PF = 2;
+        A.W = ((v >> 2) & 1) ? 16 : 8;
         A.VX = ((v >> 3) & 1) ? fp.VX / 2 : fp.VX;
     }
     A.xbase = (A.lo[0] / A.VX) * A.VX;
@@ -133,7 +134,7 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         A.nyblk = 0;
         A.nblocks = (int)(A.gx * nch);
     } else {
-        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], kWaves3D);
+        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], A.W);
         const int64_t nzc = ceil_div(A.hi[2] - A.lo[2], A.chunk);
         A.gx = 0;
         A.nblocks = (int)(A.nxseg * A.nyblk * nzc);

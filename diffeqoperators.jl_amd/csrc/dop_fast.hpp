@@ -29,6 +29,7 @@ struct FastArgs {
     int32_t K;             // window taps on every axis
     int32_t VX;            // points per lane along x (16-, 8- or 4-byte lane loads)
     int32_t PF;            // rows / planes prefetched ahead in the register ring
+    int32_t W;             // 3-D: waves (= y rows) per block
     int32_t lo[3], hi[3];  // interior box [lo, hi) per axis
     int32_t xbase;         // aligned x of the first segment
     int32_t nxseg;         // x segments (64*VX points) per row

@@ -191,6 +191,84 @@ __device__ inline const double* wtab<double>(const FastArgs& A, int a) { return 
 template <>
 __device__ inline const float* wtab<float>(const FastArgs& A, int a) { return A.w32[a]; }
 
+// lane i <- lane i-1 (wave_shr:1) / lane i <- lane i+1 (wave_shl:1); lanes without a source
+// read 0
+__device__ inline float dpp_shr1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                  0x138, 0xf, 0xf, false));
+}
+__device__ inline float dpp_shl1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                  0x130, 0xf, 0xf, false));
+}
+__device__ inline double dpp_shr1(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ inline double dpp_shl1(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// x window of a lane's VX consecutive points: xw[e] = u(xs - H + e), e < VX + 2H.  The
+// centre comes from the lane's own registers, the sides from the neighbouring lanes through
+// DPP shifts (all 64 lanes active); only the DL edge lanes of the wave load what lies
+// outside its segment.
+template <typename T, int VX, int H>
+__device__ inline void x_window(const typename Store<T>::C* own, typename Store<T>::C* xw, int lane,
+                                const T* __restrict__ prow, int64_t xs, int64_t n0)
+{
+    using C = typename Store<T>::C;
+    constexpr int DL = (H + VX - 1) / VX;   // lanes reached on each side
+#pragma unroll
+    for (int v = 0; v < VX; ++v) xw[H + v] = own[v];
+    C l[VX], r[VX];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+        l[v] = own[v];
+        r[v] = own[v];
+    }
+#pragma unroll
+    for (int d = 1; d <= DL; ++d) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+            l[v] = dpp_shr1(l[v]);   // now holds lane (lane - d)'s value v
+            r[v] = dpp_shl1(r[v]);   // lane (lane + d)'s value v
+        }
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+            const int el = H - d * VX + v;          // u(xs - d*VX + v)
+            if (el >= 0 && el < H) xw[el] = l[v];
+            const int er = H + VX + (d - 1) * VX + v;   // u(xs + d*VX + v)
+            if (er < VX + 2 * H) xw[er] = r[v];
+        }
+    }
+    if (lane < DL) {
+#pragma unroll
+        for (int e = 0; e < H; ++e) {
+            const int64_t x = xs - H + e;
+            if (x >= xs - lane * VX - H && x < xs - lane * VX)   // outside this wave's segment
+                xw[e] = (x >= 0 && x < n0) ? (C)Store<T>::ld(prow, x) : C(0);
+        }
+    }
+    if (lane >= 64 - DL) {
+#pragma unroll
+        for (int e = 0; e < H; ++e) {
+            const int64_t x = xs + VX + e;
+            if (x >= xs + (64 - lane) * VX)   // beyond the wave's segment
+                xw[H + VX + e] = (x >= 0 && x < n0) ? (C)Store<T>::ld(prow, x) : C(0);
+        }
+    }
+}
+
 // logical tile of this block: each XCD (blocks b, b+8, b+16, ...) gets a contiguous band
 __device__ inline int tile_of(int b, int nb, int remap)
 {
@@ -240,8 +318,7 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     using C = typename Store<T>::C;
     constexpr int H = K / 2;
     constexpr int R = K + PF;
-    constexpr int HA = ((H + VX - 1) / VX) * VX;
-    constexpr int NWIN = VX + 2 * HA;
+    constexpr int NWIN = VX + 2 * H;
     if ((int)blockIdx.x < nshell) {
         const int64_t per = 256 * kShellPtsPerThread;
         shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
@@ -300,23 +377,7 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
             if (y >= y1) break;
             load_row(y + H + PF, q[(j + K - 1 + PF) % R]);
             C xw[NWIN];
-#pragma unroll
-            for (int v = 0; v < VX; ++v) xw[HA + v] = q[(j + H) % R][v];
-            const T* prow = u + (int64_t)y * s1;
-            if (full) {
-#pragma unroll
-                for (int c = 0; c < HA / VX; ++c) {
-                    VecIO<T, VX>::ld(prow + xs - HA + c * VX, &xw[c * VX]);
-                    VecIO<T, VX>::ld(prow + xs + VX + c * VX, &xw[HA + VX + c * VX]);
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < HA; ++e) {
-                    const int64_t xl = xs - HA + e, xr = xs + VX + e;
-                    xw[e] = (xl >= 0 && xl < n0) ? (C)Store<T>::ld(prow, xl) : C(0);
-                    xw[HA + VX + e] = (xr >= 0 && xr < n0) ? (C)Store<T>::ld(prow, xr) : C(0);
-                }
-            }
+            x_window<T, VX, H>(q[(j + H) % R], xw, lane, u + (int64_t)y * s1, xs, n0);
             C wy[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) wy[k] = UNI ? wyu[k] : wyt[(int64_t)y * K + k];
@@ -325,7 +386,7 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
             for (int v = 0; v < VX; ++v) {
                 C s = C(0);
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[HA - H + v + k], s);
+                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[v + k], s);
 #pragma unroll
                 for (int k = 0; k < K; ++k) s = fmac<C>(wy[k], q[(j + k) % R][v], s);
                 out[v] = s;
@@ -349,10 +410,9 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
 }
 
 // ---------------------------------------------------------------------------------------
-constexpr int kW3 = 8;   // waves (= y rows) per 3-D block
-
-template <typename T, int VX, int K, bool UNI, int PF>
-__global__ __launch_bounds__(64 * kW3) void k_tiled3d(const FastArgs A, const PlanDev* __restrict__ Pp,
+// W waves (= y rows) per 3-D block
+template <typename T, int VX, int K, bool UNI, int PF, int W>
+__global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const PlanDev* __restrict__ Pp,
                                                      const ShellArgs SH, int nshell,
                                                      T* __restrict__ du, const T* __restrict__ u,
                                                      int accumulate)
@@ -360,14 +420,13 @@ __global__ __launch_bounds__(64 * kW3) void k_tiled3d(const FastArgs A, const Pl
     using C = typename Store<T>::C;
     constexpr int H = K / 2;
     constexpr int R = K + PF;
-    constexpr int HA = ((H + VX - 1) / VX) * VX;
-    constexpr int NWIN = VX + 2 * HA;
+    constexpr int NWIN = VX + 2 * H;
     constexpr int ROWW = 64 * VX;
-    constexpr int LROWS = kW3 + 2 * H;
-    static_assert(2 * H <= kW3, "halo rows are staged by one wave each");
+    constexpr int LROWS = W + 2 * H;
+    static_assert(2 * H <= W, "halo rows are staged by one wave each");
     __shared__ C lds[2][LROWS][ROWW];
     if ((int)blockIdx.x < nshell) {   // whole block: no barrier is reached
-        const int64_t per = 64 * kW3 * kShellPtsPerThread;
+        const int64_t per = 64 * W * kShellPtsPerThread;
         shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
         return;
     }
@@ -379,7 +438,7 @@ __global__ __launch_bounds__(64 * kW3) void k_tiled3d(const FastArgs A, const Pl
     const int t = b / A.nxseg;
     const int yblk = t % A.nyblk;
     const int zch = t / A.nyblk;
-    const int yb0 = A.lo[1] + yblk * kW3;
+    const int yb0 = A.lo[1] + yblk * W;
     const int y = yb0 + wave;   // this wave's output row
     const bool yok = y < A.hi[1];
     const int z0 = A.lo[2] + zch * A.chunk;
@@ -393,9 +452,9 @@ __global__ __launch_bounds__(64 * kW3) void k_tiled3d(const FastArgs A, const Pl
     const int yc = min(max(y, 0), n1 - 1);
     // halo row staged by waves 0 .. 2H-1
     const bool has_halo = wave < 2 * H;
-    const int yh = wave < H ? yb0 - H + wave : yb0 + kW3 + (wave - H);
+    const int yh = wave < H ? yb0 - H + wave : yb0 + W + (wave - H);
     const int yhc = min(max(yh, 0), n1 - 1);
-    const int hrow = wave < H ? wave : kW3 + wave;
+    const int hrow = wave < H ? wave : W + wave;
 
     const C* wxt = wtab<C>(A, 0);
     const C* wyt = wtab<C>(A, 1);
@@ -452,23 +511,8 @@ __global__ __launch_bounds__(64 * kW3) void k_tiled3d(const FastArgs A, const Pl
             }
             __syncthreads();
             C xw[NWIN];
-#pragma unroll
-            for (int v = 0; v < VX; ++v) xw[HA + v] = q[(j + H) % R][v];
-            const T* prow = u + (int64_t)z * s2 + (int64_t)yc * s1;
-            if (full) {
-#pragma unroll
-                for (int c = 0; c < HA / VX; ++c) {
-                    VecIO<T, VX>::ld(prow + xs - HA + c * VX, &xw[c * VX]);
-                    VecIO<T, VX>::ld(prow + xs + VX + c * VX, &xw[HA + VX + c * VX]);
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < HA; ++e) {
-                    const int64_t xl = xs - HA + e, xr = xs + VX + e;
-                    xw[e] = (xl >= 0 && xl < n0) ? (C)Store<T>::ld(prow, xl) : C(0);
-                    xw[HA + VX + e] = (xr >= 0 && xr < n0) ? (C)Store<T>::ld(prow, xr) : C(0);
-                }
-            }
+            x_window<T, VX, H>(q[(j + H) % R], xw, lane, u + (int64_t)z * s2 + (int64_t)yc * s1,
+                               xs, n0);
             C wz[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) wz[k] = UNI ? wzu[k] : wzt[(int64_t)z * K + k];
@@ -477,7 +521,7 @@ __global__ __launch_bounds__(64 * kW3) void k_tiled3d(const FastArgs A, const Pl
             for (int v = 0; v < VX; ++v) {
                 C s = C(0);
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[HA - H + v + k], s);
+                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[v + k], s);
 #pragma unroll
                 for (int k = 0; k < K; ++k) s = fmac<C>(wy[k], lds[buf][wave + k][lane * VX + v], s);
 #pragma unroll
@@ -512,7 +556,7 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
 {
     const FastPlan& fp = p.fast;
     const bool uni = A.uniform[0] && A.uniform[1] && (fp.kind == 2 || A.uniform[2]);
-    const int threads = fp.kind == 2 ? 256 : 64 * kW3;
+    const int threads = fp.kind == 2 ? 256 : 64 * A.W;
     const int64_t per = (int64_t)threads * kShellPtsPerThread;
     const int nshell = (int)((fp.shell_total + per - 1) / per);
     const dim3 grid((unsigned)(A.nblocks + nshell));
@@ -528,13 +572,13 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
 #undef DOP_L2
     } else {
         if constexpr (K <= 9 && VX <= 4) {
-#define DOP_L3(UNI, PF)                                                                      \
-    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, PF>), grid, dim3(64 * kW3), 0, s, A, p.dev,   \
+#define DOP_L3(UNI, W)                                                                       \
+    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, 2, W>), grid, dim3(64 * W), 0, s, A, p.dev,   \
                        fp.shell, nshell, (T*)du, (const T*)u, accumulate)
             if (uni) {
-                if (A.PF == 4) DOP_L3(true, 4); else DOP_L3(true, 2);
+                if (A.W == 16) DOP_L3(true, 16); else DOP_L3(true, 8);
             } else {
-                if (A.PF == 4) DOP_L3(false, 4); else DOP_L3(false, 2);
+                if (A.W == 16) DOP_L3(false, 16); else DOP_L3(false, 8);
             }
 #undef DOP_L3
         } else {
