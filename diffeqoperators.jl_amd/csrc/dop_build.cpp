@@ -120,19 +120,29 @@ int build_host_plan(const dop_desc& D, HostPlan* out, std::string* err)
         AxisDev& X = T.meta;
         if (a < D.ndim && D.input == DOP_INPUT_GHOST && D.bc[a].present) {
             const dop_affine_bc& q = D.bc[a];
-            if (q.n_l < 0 || q.n_r < 0 || q.n_l > DOP_MAX_BC_TAPS || q.n_r > DOP_MAX_BC_TAPS ||
+            if (q.kind != DOP_BC_AFFINE && q.kind != DOP_BC_PERIODIC && q.kind != DOP_BC_PERIODIC_ND) {
+                *err = "unknown boundary condition kind";
+                return DOP_EINVAL;
+            }
+            X.periodic = q.kind;
+            if (q.kind != DOP_BC_AFFINE) {
+                X.nl = X.nr = 0;
+                X.bl = X.br = 0.0;
+                for (int k = 0; k < DOP_MAX_BC_TAPS; ++k) X.al[k] = X.ar[k] = 0.0;
+            } else if (q.n_l < 0 || q.n_r < 0 || q.n_l > DOP_MAX_BC_TAPS || q.n_r > DOP_MAX_BC_TAPS ||
                 q.n_l > P.n[a] || q.n_r > P.n[a]) {
                 *err = "boundary condition has more taps than the axis has points";
                 return DOP_EINVAL;
+            } else {
+                X.nl = q.n_l;
+                X.nr = q.n_r;
+                for (int k = 0; k < DOP_MAX_BC_TAPS; ++k) {
+                    X.al[k] = k < q.n_l ? q.a_l[k] : 0.0;
+                    X.ar[k] = k < q.n_r ? q.a_r[k] : 0.0;
+                }
+                X.bl = q.b_l;
+                X.br = q.b_r;
             }
-            X.nl = q.n_l;
-            X.nr = q.n_r;
-            for (int k = 0; k < DOP_MAX_BC_TAPS; ++k) {
-                X.al[k] = k < q.n_l ? q.a_l[k] : 0.0;
-                X.ar[k] = k < q.n_r ? q.a_r[k] : 0.0;
-            }
-            X.bl = q.b_l;
-            X.br = q.b_r;
         }
         int64_t ts = 0;
         if (D.input == DOP_INPUT_PADDED) {

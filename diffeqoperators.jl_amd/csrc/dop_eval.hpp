@@ -120,6 +120,7 @@ struct Acc<kAccF32Kahan> {
 template <typename T>
 DOP_HD inline double ghost_left(const AxisDev& X, const T* src, int64_t base, int64_t S)
 {
+    if (X.periodic) return (double)Store<T>::ld(src, base + (X.periodic == 1 ? (int64_t)(X.n - 1) * S : 0));
     double g = 0.0;
     for (int k = 0; k < X.nl; ++k) g = fma(X.al[k], (double)Store<T>::ld(src, base + k * S), g);
     return g + X.bl;
@@ -128,6 +129,7 @@ DOP_HD inline double ghost_left(const AxisDev& X, const T* src, int64_t base, in
 template <typename T>
 DOP_HD inline double ghost_right(const AxisDev& X, const T* src, int64_t base, int64_t S)
 {
+    if (X.periodic) return (double)Store<T>::ld(src, base + (X.periodic == 2 ? (int64_t)(X.n - 1) * S : 0));
     double g = 0.0;
     const int64_t o = (int64_t)X.n - X.nr;
     for (int k = 0; k < X.nr; ++k)

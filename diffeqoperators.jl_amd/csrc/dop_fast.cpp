@@ -49,11 +49,22 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     // points per lane: 16-byte lane loads when rows stay aligned
     const int vx_native = (P.storage == kF64) ? 2 : 4;
     fp->VX = (P.n[0] % vx_native == 0) ? vx_native : 1;
-    // padded weights (rows outside the box are zero)
+    if (P.ndim == 3) {
+        // 3-D lanes store whole VX groups: the box's x edges are VX-aligned (the few columns
+        // this moves into the shell are evaluated there)
+        A.lo[0] = (int)ceil_div(A.lo[0], fp->VX) * fp->VX;
+        A.hi[0] = A.hi[0] / fp->VX * fp->VX;
+        if (A.hi[0] <= A.lo[0]) return;
+    }
+    // padded weights (rows outside the box are zero).  The tiled kernels index every axis
+    // per row unless all axes are uniform, so a uniform axis of a mixed plan is expanded to
+    // one row per point.
+    bool all_uniform = true;
+    for (int a = 0; a < P.ndim; ++a) all_uniform = all_uniform && P.ax[a].uniform;
     for (int a = 0; a < P.ndim; ++a) {
         const AxisDev& X = P.ax[a];
         const AxisTables& T = H.axis[a];
-        const int rows = X.uniform ? 1 : (int)P.n[a];
+        const int rows = all_uniform ? 1 : (int)P.n[a];
         fp->w64[a].assign((size_t)rows * K, 0.0);
         for (int r = 0; r < rows; ++r) {
             const int src = X.uniform ? 0 : r;
@@ -122,19 +133,29 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
     } else {
         static const int chunks[2] = {64, 32};
         A.chunk = chunks[(v >> 1) & 1];
-        A.PF = 2;
+        A.PF = 1;
         A.W = ((v >> 2) & 1) ? 16 : 8;
+        if (2 * (A.K / 2) >= 8) A.W = 16;   // an 8-wave block must compute at least two rows
         A.VX = ((v >> 3) & 1) ? fp.VX / 2 : fp.VX;
     }
-    A.xbase = (A.lo[0] / A.VX) * A.VX;
-    A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, 64 * A.VX);
+    if (fp.kind == 2) {
+        A.xbase = (A.lo[0] / A.VX) * A.VX;
+        A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, 64 * A.VX);
+    } else {
+        // segments of 64*VX points overlapping by 2*PADL (the x halo, whole lanes)
+        const int H = A.K / 2;
+        const int padl = (H + A.VX - 1) / A.VX * A.VX;
+        A.xbase = A.lo[0] - padl;
+        A.nxseg = (int)ceil_div(A.hi[0] - A.lo[0], 64 * A.VX - 2 * padl);
+    }
     if (fp.kind == 2) {
         A.gx = (int)ceil_div(A.nxseg, 4);
         const int64_t nch = ceil_div(A.hi[1] - A.lo[1], A.chunk);
         A.nyblk = 0;
         A.nblocks = (int)(A.gx * nch);
     } else {
-        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], A.W);
+        // W waves stage W rows and compute the middle W - 2H
+        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], A.W - 2 * (A.K / 2));
         const int64_t nzc = ceil_div(A.hi[2] - A.lo[2], A.chunk);
         A.gx = 0;
         A.nblocks = (int)(A.nxseg * A.nyblk * nzc);

@@ -31,8 +31,9 @@ struct FastArgs {
     int32_t PF;            // rows / planes prefetched ahead in the register ring
     int32_t W;             // 3-D: waves (= y rows) per block
     int32_t lo[3], hi[3];  // interior box [lo, hi) per axis
-    int32_t xbase;         // aligned x of the first segment
-    int32_t nxseg;         // x segments (64*VX points) per row
+    int32_t xbase;         // x of the first segment's first point (VX-aligned; 3-D: may be
+                           // negative, the segment's first PADL points are x halo)
+    int32_t nxseg;         // x segments (64*VX points; 3-D: overlapping by 2*PADL) per row
     int32_t chunk;         // rows (2-D) / planes (3-D) per block
     int32_t nyblk;         // 3-D: y blocks of W rows
     int32_t gx;            // 2-D: blocks across x (4 segments each)

@@ -75,7 +75,9 @@ static hipError_t launch_generic_any(const PlanDev& host, const PlanDev* dev, vo
 hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumulate,
                         int variant, int force_generic, hipStream_t s)
 {
-    if (p.fast.kind != 0 && !p.fast.kahan && !force_generic)
+    // the tiled kernels use 16-byte lane loads/stores: both arrays must be 16-byte aligned
+    const bool aligned = ((uintptr_t)du % 16 == 0) && ((uintptr_t)u % 16 == 0);
+    if (p.fast.kind != 0 && !p.fast.kahan && !force_generic && aligned)
         return launch_tiled(p, du, u, accumulate, variant < 0 ? p.variant : variant, s);
     return launch_generic_any(p.host, p.dev, du, u, accumulate, s);
 }

@@ -37,7 +37,8 @@ struct AxisDev {
     int32_t KB;       // max taps of a boundary row
     int32_t nb;       // number of boundary rows = rlo + (n - rhi)
     int32_t nl, nr;   // ghost taps
-    int32_t pad_;
+    int32_t periodic; // 0: affine ghosts; 1: ghost_l = u[n-1], ghost_r = u[0];
+                      // 2: ghost_l = u[0], ghost_r = u[n-1]
     double al[kMaxBCTaps];
     double ar[kMaxBCTaps];
     double bl, br;

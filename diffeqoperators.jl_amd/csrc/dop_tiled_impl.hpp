@@ -5,10 +5,10 @@
 //      y stencil live in a register ring of K+PF rows (PF rows prefetched ahead), so every
 //      row of u is loaded once per chunk; the x window of the centre row comes from lane
 //      loads that hit L1 (the row was loaded by the same wave PF+H steps earlier).
-// 3-D: a block of 8 waves owns 8 consecutive y rows of one x segment and marches a chunk
+// 3-D: a block of W waves owns W consecutive y rows of one x segment and marches a chunk
 //      of z planes with the same register ring along z.  Each step stages the centre plane
-//      (own rows + 2H halo rows) in LDS, double-buffered, one barrier per step; y
-//      neighbours are read from LDS, x neighbours from L1.
+//      (own rows + 2H halo rows, each with H x-halo columns) in LDS, double-buffered, one
+//      barrier per step; x and y neighbours are read from LDS, z neighbours from the ring.
 // Both sum, per point, the x window, then y, then z, each in tap order: the same FMA
 // sequence as the generic path, so the two agree bitwise on finite inputs.
 //
@@ -269,6 +269,19 @@ __device__ inline void x_window(const typename Store<T>::C* own, typename Store<
     }
 }
 
+// Calls f(std::integral_constant<int, J>) for J = J0 .. R-1 until f returns false: the
+// steps of a register ring with compile-time slot indices, whatever the body's size.
+template <int J, int R, typename F>
+__device__ __forceinline__ bool ring_steps(F& f)
+{
+    if constexpr (J == R) {
+        return true;
+    } else {
+        if (!f(std::integral_constant<int, J>{})) return false;
+        return ring_steps<J + 1, R>(f);
+    }
+}
+
 // logical tile of this block: each XCD (blocks b, b+8, b+16, ...) gets a contiguous band
 __device__ inline int tile_of(int b, int nb, int remap)
 {
@@ -410,7 +423,24 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
 }
 
 // ---------------------------------------------------------------------------------------
-// W waves (= y rows) per 3-D block
+// 3-D: a block of W waves stages W consecutive y rows of one x segment and computes the
+// middle W - 2H of them (the first and last H waves load halo rows only).  Every wave walks
+// a chunk of z planes with a register ring of K + PF planes of its row, loaded PF + H
+// planes ahead, and each step copies its centre plane into LDS (double-buffered, one
+// barrier per step), so every staged value was loaded several steps before it is used.
+// A wave loads an aligned 64*VX-point row segment whose first and last PADL points (whole
+// lanes) are the x halo: segments overlap by 2*PADL and only the middle lanes store.  The
+// memory operations of a step are the same on every path (addresses are clamped into the
+// array and out-of-range values selected to zero), which keeps the hardware's in-order
+// load counters precise so that a wait never drains the prefetched ring.  x and y taps come
+// from LDS, z taps from the ring; non-uniform x weights (per column) and z weights (per
+// plane of the chunk) are staged in LDS once.
+constexpr int kMaxChunk3D = 64;
+// planes prefetched beyond the deepest z tap (the ring holds K + PF planes), as deep as the
+// registers allow without spills
+template <typename T, int K>
+constexpr int kPF3D = sizeof(T) == 8 ? 1 : K <= 3 ? 4 : K <= 7 ? 2 : 1;
+
 template <typename T, int VX, int K, bool UNI, int PF, int W>
 __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const PlanDev* __restrict__ Pp,
                                                      const ShellArgs SH, int nshell,
@@ -420,11 +450,15 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     using C = typename Store<T>::C;
     constexpr int H = K / 2;
     constexpr int R = K + PF;
-    constexpr int NWIN = VX + 2 * H;
     constexpr int ROWW = 64 * VX;
-    constexpr int LROWS = W + 2 * H;
-    static_assert(2 * H <= W, "halo rows are staged by one wave each");
-    __shared__ C lds[2][LROWS][ROWW];
+    constexpr int PADL = (H + VX - 1) / VX * VX;          // x halo, whole lanes
+    constexpr int LEDGE = PADL / VX;                      // load-only lanes at each end
+    constexpr int NRD = (PADL + H + VX + VX - 1) / VX;    // VX-wide reads covering a window
+    static_assert(2 * H < W, "a block computes at least one row");
+    static_assert((64 - 2 * LEDGE - 1) * VX + NRD * VX <= ROWW, "x window inside the row");
+    __shared__ __attribute__((aligned(16))) C lds[2][W][ROWW];
+    __shared__ C wzs[UNI ? 1 : kMaxChunk3D][K];
+    __shared__ __attribute__((aligned(16))) C wxs[UNI ? 1 : K][UNI ? 1 : ROWW];   // per column
     if ((int)blockIdx.x < nshell) {   // whole block: no barrier is reached
         const int64_t per = 64 * W * kShellPtsPerThread;
         shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
@@ -438,114 +472,121 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     const int t = b / A.nxseg;
     const int yblk = t % A.nyblk;
     const int zch = t / A.nyblk;
-    const int yb0 = A.lo[1] + yblk * W;
-    const int y = yb0 + wave;   // this wave's output row
-    const bool yok = y < A.hi[1];
+    const int y = A.lo[1] + yblk * (W - 2 * H) - H + wave;   // the row this wave stages
+    const bool computes = wave >= H && wave < W - H;         // wave-uniform
+    const bool yok = computes && y < A.hi[1];
     const int z0 = A.lo[2] + zch * A.chunk;
     const int z1 = min(z0 + A.chunk, A.hi[2]);
     const int64_t n0 = A.n[0];
     const int n1 = (int)A.n[1], n2 = (int)A.n[2];
     const int64_t s1 = A.s1, s2 = A.s2;
-    const int64_t xs = (int64_t)A.xbase + (int64_t)xseg * ROWW + lane * VX;
-    const bool rowok = xs + VX <= n0;
-    const bool full = xs >= A.lo[0] && xs + VX <= A.hi[0];
+    const int64_t xs0 = (int64_t)A.xbase + (int64_t)xseg * (ROWW - 2 * PADL);
+    const int64_t xs = xs0 + lane * VX;
+    // Loads are clamped into the array.  A lane whose segment lies outside [0, n0) (or a
+    // row / plane past the edge) then holds other points' values: only outputs in the box
+    // [lo, hi) are stored, and their x / y / z windows reach no further than H points past the
+    // box, i.e. never outside the array, so those values are never used.
+    const int64_t xl = min(max(xs, (int64_t)0), n0 - VX);  // clamped load column (rows are
+                                                           // whole VX groups)
+    const bool st_ok = yok && lane >= LEDGE && lane < 64 - LEDGE && xs >= A.lo[0] && xs < A.hi[0];
     const int yc = min(max(y, 0), n1 - 1);
-    // halo row staged by waves 0 .. 2H-1
-    const bool has_halo = wave < 2 * H;
-    const int yh = wave < H ? yb0 - H + wave : yb0 + W + (wave - H);
-    const int yhc = min(max(yh, 0), n1 - 1);
-    const int hrow = wave < H ? wave : W + wave;
+    const int lr = min(max(lane, LEDGE), 63 - LEDGE);      // window reads of load-only lanes
+    const int wcol = (lr - LEDGE) * VX;                     // first column of the window reads
 
     const C* wxt = wtab<C>(A, 0);
     const C* wyt = wtab<C>(A, 1);
     const C* wzt = wtab<C>(A, 2);
-    C wx[VX][K];
-#pragma unroll
-    for (int v = 0; v < VX; ++v) {
-        int64_t xr = 0;
-        if (!UNI) xr = min(max(xs + v, (int64_t)0), n0 - 1);
-#pragma unroll
-        for (int k = 0; k < K; ++k) wx[v][k] = wxt[xr * K + k];
-    }
+    C wxu[K], wzu[K];
     C wy[K];
     {
         const int64_t yr = UNI ? 0 : yc;
 #pragma unroll
         for (int k = 0; k < K; ++k) wy[k] = wyt[yr * K + k];
     }
-    C wzu[K];
-    if (UNI) {
+    if constexpr (UNI) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) wzu[k] = wzt[k];
+        for (int k = 0; k < K; ++k) {
+            wxu[k] = wxt[k];
+            wzu[k] = wzt[k];
+        }
+    } else {
+        // visible to every wave after the first step's barrier
+        for (int i = threadIdx.x; i < (z1 - z0) * K; i += 64 * W)
+            wzs[i / K][i % K] = wzt[(int64_t)(z0 + i / K) * K + i % K];
+        for (int i = threadIdx.x; i < ROWW * K; i += 64 * W) {
+            const int c = i / K, k = i % K;
+            const int64_t x = min(max(xs0 + c, (int64_t)0), n0 - 1);
+            wxs[k][c] = wxt[x * K + k];
+        }
     }
 
-    auto load_at = [&](int yy, int zz, C* dst) {
+    // one VX-wide load per call, on every path, straight into its ring slot
+    const T* urow = u + (int64_t)yc * s1 + xl;
+    auto load_plane = [&](int zz, C* dst) {
         zz = min(max(zz, 0), n2 - 1);
-        const T* p = u + (int64_t)zz * s2 + (int64_t)yy * s1 + xs;
-        if (rowok) {
-            VecIO<T, VX>::ld(p, dst);
-        } else {
-#pragma unroll
-            for (int v = 0; v < VX; ++v) dst[v] = (xs + v < n0) ? (C)Store<T>::ld(p, v) : C(0);
-        }
+        VecIO<T, VX>::ld(urow + (int64_t)zz * s2, dst);
     };
 
     C q[R][VX];
 #pragma unroll
-    for (int s = 0; s < R - 1; ++s) load_at(yc, z0 - H + s, q[s]);
+    for (int s = 0; s < R - 1; ++s) load_plane(z0 - H + s, q[s]);
 
     int buf = 0;
     for (int zb = z0; zb < z1; zb += R) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
+        auto step = [&](auto jc) -> bool {
+            constexpr int j = decltype(jc)::value;
             const int z = zb + j;
-            if (z >= z1) break;   // block-uniform
-            load_at(yc, z + H + PF, q[(j + K - 1 + PF) % R]);
-            C hv[VX];
-            if (has_halo) load_at(yhc, z, hv);
+            if (z >= z1) return false;   // block-uniform
+            load_plane(z + H + PF, q[(j + K - 1 + PF) % R]);
 #pragma unroll
-            for (int v = 0; v < VX; ++v) lds[buf][H + wave][lane * VX + v] = q[(j + H) % R][v];
-            if (has_halo) {
-#pragma unroll
-                for (int v = 0; v < VX; ++v) lds[buf][hrow][lane * VX + v] = hv[v];
-            }
+            for (int v = 0; v < VX; ++v) lds[buf][wave][lane * VX + v] = q[(j + H) % R][v];
             __syncthreads();
-            C xw[NWIN];
-            x_window<T, VX, H>(q[(j + H) % R], xw, lane, u + (int64_t)z * s2 + (int64_t)yc * s1,
-                               xs, n0);
-            C wz[K];
+            if (computes) {
+                C xw[NRD * VX];
 #pragma unroll
-            for (int k = 0; k < K; ++k) wz[k] = UNI ? wzu[k] : wzt[(int64_t)z * K + k];
-            C out[VX];
+                for (int r = 0; r < NRD; ++r) {
 #pragma unroll
-            for (int v = 0; v < VX; ++v) {
-                C s = C(0);
+                    for (int v = 0; v < VX; ++v) xw[r * VX + v] = lds[buf][wave][wcol + r * VX + v];
+                }
+                C wz[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[v + k], s);
+                for (int k = 0; k < K; ++k) {
+                    if constexpr (UNI) wz[k] = wzu[k];
+                    else wz[k] = wzs[z - z0][k];
+                }
+                C out[VX];
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fmac<C>(wy[k], lds[buf][wave + k][lane * VX + v], s);
+                for (int v = 0; v < VX; ++v) {
+                    C s = C(0);
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[(j + k) % R][v], s);
-                out[v] = s;
-            }
-            if (yok) {
-                T* po = du + (int64_t)z * s2 + (int64_t)y * s1 + xs;
-                if (full && !accumulate) {
-                    VecIO<T, VX>::st(po, out);
-                } else {
-#pragma unroll
-                    for (int v = 0; v < VX; ++v) {
-                        const int64_t x = xs + v;
-                        if (x >= A.lo[0] && x < A.hi[0]) {
-                            double r = (double)out[v];
-                            if (accumulate) r += (double)Store<T>::ld(po, v);
-                            Store<T>::st(po, v, r);
-                        }
+                    for (int k = 0; k < K; ++k) {
+                        C w;
+                        if constexpr (UNI) w = wxu[k];
+                        else w = wxs[k][lane * VX + v];
+                        s = fmac<C>(w, xw[PADL - H + v + k], s);
                     }
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        s = fmac<C>(wy[k], lds[buf][wave - H + k][lane * VX + v], s);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[(j + k) % R][v], s);
+                    out[v] = s;
+                }
+                if (st_ok) {
+                    T* po = du + (int64_t)z * s2 + (int64_t)y * s1 + xs;
+                    if (accumulate) {
+                        C old[VX];
+                        VecIO<T, VX>::ld(po, old);
+#pragma unroll
+                        for (int v = 0; v < VX; ++v) out[v] = (C)((double)out[v] + (double)old[v]);
+                    }
+                    VecIO<T, VX>::st(po, out);
                 }
             }
             buf ^= 1;
-        }
+            return true;
+        };
+        if (!ring_steps<0, R>(step)) break;
     }
 }
 
@@ -556,6 +597,9 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
 {
     const FastPlan& fp = p.fast;
     const bool uni = A.uniform[0] && A.uniform[1] && (fp.kind == 2 || A.uniform[2]);
+    // per-row indexing needs a row of weights for every point of every axis
+    for (int a = 0; a < fp.kind; ++a)
+        if (fp.w64[a].size() != (size_t)(uni ? 1 : A.n[a]) * K) return hipErrorInvalidValue;
     const int threads = fp.kind == 2 ? 256 : 64 * A.W;
     const int64_t per = (int64_t)threads * kShellPtsPerThread;
     const int nshell = (int)((fp.shell_total + per - 1) / per);
@@ -571,14 +615,25 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
         }
 #undef DOP_L2
     } else {
-        if constexpr (K <= 9 && VX <= 4) {
+        if constexpr (K <= 9 && VX <= 16 / (int)sizeof(T) && VX <= 4) {
 #define DOP_L3(UNI, W)                                                                       \
-    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, 2, W>), grid, dim3(64 * W), 0, s, A, p.dev,   \
+    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, kPF3D<T, K>, W>), grid, dim3(64 * W), 0, s, A,    \
+                       p.dev,                                                                  \
                        fp.shell, nshell, (T*)du, (const T*)u, accumulate)
+            constexpr bool w8 = 2 * (K / 2) < 8;   // 8-wave blocks compute 8 - 2H rows
+            if (A.W != 16 && !w8) return hipErrorInvalidValue;
             if (uni) {
-                if (A.W == 16) DOP_L3(true, 16); else DOP_L3(true, 8);
+                if constexpr (w8) {
+                    if (A.W == 8) DOP_L3(true, 8); else DOP_L3(true, 16);
+                } else {
+                    DOP_L3(true, 16);
+                }
             } else {
-                if (A.W == 16) DOP_L3(false, 16); else DOP_L3(false, 8);
+                if constexpr (w8) {
+                    if (A.W == 8) DOP_L3(false, 8); else DOP_L3(false, 16);
+                } else {
+                    DOP_L3(false, 16);
+                }
             }
 #undef DOP_L3
         } else {

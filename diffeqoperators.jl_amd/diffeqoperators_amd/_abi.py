@@ -20,6 +20,7 @@ INPUT_GHOST, INPUT_PADDED = 0, 1
 SEM_LINES, SEM_BPV = 0, 1
 COEFF_VECTOR, COEFF_NUM_POS, COEFF_NUM_NEG = 0, 1, 2
 MAX_BC_TAPS = 8
+BC_AFFINE, BC_PERIODIC, BC_PERIODIC_ND = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -38,7 +39,7 @@ class DopOperator(C.Structure):
 
 class DopAffineBC(C.Structure):
     _fields_ = [
-        ("present", C.c_int32), ("n_l", C.c_int32), ("n_r", C.c_int32), ("reserved", C.c_int32),
+        ("present", C.c_int32), ("n_l", C.c_int32), ("n_r", C.c_int32), ("kind", C.c_int32),
         ("a_l", C.c_double * MAX_BC_TAPS), ("b_l", C.c_double),
         ("a_r", C.c_double * MAX_BC_TAPS), ("b_r", C.c_double),
     ]

@@ -12,7 +12,7 @@ import ctypes as C
 import torch
 
 from . import _abi
-from .bcs import AffineBC, ComposedMultiDimBC, MultiDimDirectionalBC
+from .bcs import AffineBC, ComposedMultiDimBC, MultiDimDirectionalBC, PeriodicAtomicBC
 from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
 from .operators import DerivativeOperator
 
@@ -111,6 +111,9 @@ def make_desc(terms, jsize, storage, *, input_kind, semantics, bcs=None, padded=
     d.ops = ops
     for ax, bc in (bcs or {}).items():
         q = d.bc[ax]
+        if isinstance(bc, _PeriodicPad):
+            q.present, q.kind, q.n_l, q.n_r = 1, bc.kind, 0, 0
+            continue
         if len(bc.a_l) > _abi.MAX_BC_TAPS or len(bc.a_r) > _abi.MAX_BC_TAPS:
             raise ValueError("boundary condition order too high")
         q.present = 1
@@ -126,24 +129,48 @@ def make_desc(terms, jsize, storage, *, input_kind, semantics, bcs=None, padded=
 # ------------------------------------------------------------------------------------------
 # Q*u: which BC pads which axis, and which reference kernels apply
 
+class _PeriodicPad:
+    """Periodic padding of one axis: kind BC_PERIODIC (line wrap, bc_operators.jl:192) or
+    BC_PERIODIC_ND (first/last planes, multi_dim_bc_operators.jl:221-252)."""
+
+    def __init__(self, kind):
+        self.kind = kind
+
+    def key(self):
+        return ("periodic", self.kind)
+
+
 def _bcs_of(Q, ndim, jsize):
-    """{axis0: AffineBC} of Q applied to an array of Julia size jsize, and the semantics."""
-    if isinstance(Q, AffineBC):
-        # 1-D: Q*u is a BoundaryPaddedVector (bc_operators.jl:188-191);
-        # N-D: BC*u = MultiDimBC{1}(BC, size(u))*u pads axis 1 (multi_dim_bc_operators.jl:210)
+    """{axis0: AffineBC | _PeriodicPad} of Q applied to an array of Julia size jsize, and the
+    semantics."""
+    if isinstance(Q, (AffineBC, PeriodicAtomicBC)):
+        # 1-D: Q*u is a BoundaryPaddedVector (bc_operators.jl:188-192);
+        # N-D: BC*u = MultiDimBC{1}(BC, size(u))*u pads axis 1 (multi_dim_bc_operators.jl:210),
+        # which for PeriodicBC takes the N-D periodic method (:221-228)
+        if isinstance(Q, PeriodicAtomicBC):
+            Q = _PeriodicPad(_abi.BC_PERIODIC if ndim == 1 else _abi.BC_PERIODIC_ND)
         return {0: Q}, (_abi.SEM_BPV if ndim == 1 else _abi.SEM_LINES)
     if ndim == 1:
         raise AssertionError("multi-dimensional BCs need arrays of 2 or more dimensions")
     if isinstance(Q, MultiDimDirectionalBC):
         if tuple(Q.s) != tuple(jsize):
             raise AssertionError("Size of the BCs array in the MultiDimBC is incorrect")
-        return {Q.D - 1: Q.bc}, _abi.SEM_LINES
+        bc = Q.bc
+        if isinstance(bc, PeriodicAtomicBC):
+            bc = _PeriodicPad(_abi.BC_PERIODIC_ND)
+        return {Q.D - 1: bc}, _abi.SEM_LINES
     if isinstance(Q, ComposedMultiDimBC):
+        # all atoms periodic: the N-D periodic method (:244-252); otherwise every line is
+        # padded by its own atom (slice_rmul), i.e. the line wrap for periodic atoms
+        all_periodic = all(isinstance(b.bc, PeriodicAtomicBC) for b in Q.bcs)
         out = {}
         for b in Q.bcs:
             if tuple(b.s) != tuple(jsize):
                 raise AssertionError(f"Size of the BCs array for dimension {b.D} is incorrect")
-            out[b.D - 1] = b.bc
+            bc = b.bc
+            if isinstance(bc, PeriodicAtomicBC):
+                bc = _PeriodicPad(_abi.BC_PERIODIC_ND if all_periodic else _abi.BC_PERIODIC)
+            out[b.D - 1] = bc
         return out, _abi.SEM_LINES
     raise TypeError(f"not a boundary condition: {type(Q)}")
 

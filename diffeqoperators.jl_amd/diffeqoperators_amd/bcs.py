@@ -4,7 +4,10 @@ multi_dim_bc_operators.jl.
 
 An affine BC pads a line u with ghosts l = a_l . u[1:k] + b_l and r = a_r . u[end-k+1:end]
 + b_r (bc_operators.jl:188-191).  Dirichlet is Robin with beta = 0 evaluated through IEEE
-division by zero (a = +-0.0), exactly as in the reference (:176-178).
+division by zero (a = +-0.0), exactly as in the reference (:176-178).  PeriodicBC pads a line
+with u[end], u[1] (:192); extended over an N-D array by a directional or composed BC whose
+atoms are all periodic, its ghost planes are the first and last planes of u instead
+(multi_dim_bc_operators.jl:221-252).
 """
 from __future__ import annotations
 
@@ -29,6 +32,18 @@ class AffineBC:
 
     def __repr__(self):
         return f"AffineBC(a_l={self.a_l.tolist()}, b_l={self.b_l}, a_r={self.a_r.tolist()}, b_r={self.b_r})"
+
+
+class PeriodicAtomicBC:
+    """PeriodicBC(T) (bc_operators.jl:17-19)."""
+
+    periodic = True
+
+    def key(self):
+        return ("periodic",)
+
+    def __repr__(self):
+        return "PeriodicBC()"
 
 
 def _robin_atomic(l, r, dx, order=1):
@@ -106,7 +121,7 @@ def decompose(Q: ComposedMultiDimBC):
 
 
 def is_bc(x) -> bool:
-    return isinstance(x, (AffineBC, MultiDimDirectionalBC, ComposedMultiDimBC))
+    return isinstance(x, (AffineBC, PeriodicAtomicBC, MultiDimDirectionalBC, ComposedMultiDimBC))
 
 
 def _is_size(s):
@@ -197,8 +212,19 @@ Neumann0BC = _DimBC(_neumann0, _neumann0_dim)
 MultiDimBC = _DimBC(_multidim, _multidim_dim)
 
 
-def PeriodicBC(*args, **kwargs):
-    raise NotImplementedError("PeriodicBC is not on the accelerated path of this library")
+def _periodic(T=float, s=None):
+    if s is None:
+        return PeriodicAtomicBC()
+    # PeriodicBC(T, s) = MultiDimBC(PeriodicBC(T), s)  (multi_dim_bc_operators.jl:114)
+    return _multidim(PeriodicAtomicBC(), s)
+
+
+def _periodic_dim(dim, T, s):
+    # PeriodicBC{dim}(T, s) = MultiDimBC{dim}(PeriodicBC(T), s)  (:113)
+    return MultiDimDirectionalBC(PeriodicAtomicBC(), dim, s)
+
+
+PeriodicBC = _DimBC(_periodic, _periodic_dim)
 
 
 def GeneralBC(*args, **kwargs):

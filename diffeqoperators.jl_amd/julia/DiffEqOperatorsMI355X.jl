@@ -11,7 +11,7 @@ module DiffEqOperatorsMI355X
 
 using DiffEqOperators, LinearAlgebra
 import DiffEqOperators: DerivativeOperator, GhostDerivativeOperator, DiffEqOperatorCombination,
-                        RobinBC, GeneralBC, MultiDimDirectionalBC, ComposedMultiDimBC,
+                        AtomicBC, AffineBC, PeriodicBC, MultiDimDirectionalBC, ComposedMultiDimBC,
                         use_winding, diff_axis
 
 const libdop = get(ENV, "DOP_LIBRARY", joinpath(@__DIR__, "..", "lib", "libdop.so"))
@@ -27,7 +27,7 @@ struct DopOperator
 end
 
 struct DopAffineBC
-    present::Int32; n_l::Int32; n_r::Int32; reserved::Int32
+    present::Int32; n_l::Int32; n_r::Int32; kind::Int32
     a_l::NTuple{8, Float64}; b_l::Float64; a_r::NTuple{8, Float64}; b_r::Float64
 end
 
@@ -78,13 +78,23 @@ function dop_operator(L::DerivativeOperator, keep)
 end
 
 pad8(v) = ntuple(i -> i <= length(v) ? Float64(v[i]) : 0.0, 8)
-dop_bc(q) = DopAffineBC(1, length(q.a_l), length(q.a_r), 0, pad8(q.a_l), q.b_l, pad8(q.a_r), q.b_r)
+const DOP_BC_AFFINE, DOP_BC_PERIODIC, DOP_BC_PERIODIC_ND = 0, 1, 2
+dop_bc(q::AffineBC) = DopAffineBC(1, length(q.a_l), length(q.a_r), DOP_BC_AFFINE, pad8(q.a_l), q.b_l,
+                                  pad8(q.a_r), q.b_r)
+# kind: DOP_BC_PERIODIC for Q*u on a line (bc_operators.jl:192), DOP_BC_PERIODIC_ND for the
+# N-D padding of an all-periodic MultiDimDirectionalBC / ComposedMultiDimBC
+# (multi_dim_bc_operators.jl:221-252)
+dop_bc(q::PeriodicBC, kind) = DopAffineBC(1, 0, 0, kind, pad8(()), 0.0, pad8(()), 0.0)
+dop_bc(q::AffineBC, kind) = dop_bc(q)
 const NO_BC = DopAffineBC(0, 0, 0, 0, pad8(()), 0.0, pad8(()), 0.0)
 
-# per-axis atomic BCs of a Q applied to an array of size s
-axis_bcs(Q::Union{RobinBC, GeneralBC}, s) = Dict(1 => Q)
-axis_bcs(Q::MultiDimDirectionalBC{T, B, D}, s) where {T, B, D} = Dict(D => first(Q.BCs))
-axis_bcs(Q::ComposedMultiDimBC, s) = Dict(d => first(Q.BCs[d]) for d in 1:length(Q.BCs))
+# per-axis (atomic BC, kind) of a Q applied to an array of size s
+axis_bcs(Q::AtomicBC, s) = Dict(1 => (Q, length(s) == 1 ? DOP_BC_PERIODIC : DOP_BC_PERIODIC_ND))
+axis_bcs(Q::MultiDimDirectionalBC{T, B, D}, s) where {T, B, D} =
+    Dict(D => (first(Q.BCs), DOP_BC_PERIODIC_ND))
+axis_bcs(Q::ComposedMultiDimBC{T, B}, s) where {T, B} =
+    Dict(d => (first(Q.BCs[d]), B <: PeriodicBC ? DOP_BC_PERIODIC_ND : DOP_BC_PERIODIC)
+         for d in 1:length(Q.BCs))
 
 mutable struct Plan
     handle::Ptr{Cvoid}
@@ -97,10 +107,10 @@ function Plan(ops::Vector, Q, sz::NTuple{N, Int}, ::Type{T}; input = DOP_INPUT_G
     dops = [dop_operator(L, keep) for L in ops]
     push!(keep, dops)
     bcs = Q === nothing ? Dict() : axis_bcs(Q, sz)
-    sem = (N == 1 && Q isa Union{RobinBC, GeneralBC}) ? DOP_SEM_BPV : DOP_SEM_LINES
+    sem = (N == 1 && Q isa AtomicBC) ? DOP_SEM_BPV : DOP_SEM_LINES
     desc = DopDesc(N, storage_code(T), 0, input, sem, length(dops),
                    ntuple(i -> i <= N ? sz[i] : 1, 3), Int32.(padded), transverse_view,
-                   pointer(dops), ntuple(i -> haskey(bcs, i) ? dop_bc(bcs[i]) : NO_BC, 3))
+                   pointer(dops), ntuple(i -> haskey(bcs, i) ? dop_bc(bcs[i]...) : NO_BC, 3))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:dop_plan_create, libdop), Cint, (Ref{DopDesc}, Ref{Ptr{Cvoid}}), desc, h))
     p = Plan(h[], keep)

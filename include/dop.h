@@ -102,10 +102,17 @@ typedef struct dop_operator {
  * ghost_r = a_r . u[end-n_r+1:end] + b_r, identical for every line of that axis
  * (multi_dim_bc_operators.jl:141-149). */
 #define DOP_MAX_BC_TAPS 8
+#define DOP_BC_AFFINE 0        /* ghosts a . u + b as above */
+#define DOP_BC_PERIODIC 1      /* PeriodicBC on a line: ghost_l = u[end], ghost_r = u[1]
+                                  (bc_operators.jl:192); taps ignored */
+#define DOP_BC_PERIODIC_ND 2   /* PeriodicBC extended over an N-D array (MultiDimDirectionalBC /
+                                  ComposedMultiDimBC of PeriodicBC): ghost planes are the first
+                                  and last planes, ghost_l = u[1], ghost_r = u[end]
+                                  (multi_dim_bc_operators.jl:221-252); taps ignored */
 typedef struct dop_affine_bc {
     int32_t present;                  /* 0: axis has no BC (must carry no operator) */
     int32_t n_l, n_r;
-    int32_t reserved;
+    int32_t kind;                     /* DOP_BC_* */
     double a_l[DOP_MAX_BC_TAPS];
     double b_l;
     double a_r[DOP_MAX_BC_TAPS];

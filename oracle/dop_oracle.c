@@ -694,12 +694,21 @@ int orc_ghost_apply(int nops, const orc_op* ops, int ndim, const long long* size
                 const long long b = line_base(ndim, size, idx, zero, ax);
                 /* u[I, :, J] copied (multi_dim_bc_operators.jl:34) */
                 for (int j = 0; j < len; ++j) xl[j + 1] = u[b + (long long)j * stride];
-                /* Q*u_line: bc_operators.jl:188-191 (a_l . u[1:nl] + b_l) */
+                /* Q*u_line: bc_operators.jl:188-191 (a_l . u[1:nl] + b_l), or the
+                 * periodic wrap of :192 (u[end], u[1]) */
                 double gl = 0.0, gr = 0.0;
-                for (int k = 0; k < q->nl; ++k) gl += q->a_l[k] * xl[1 + k];
-                for (int k = 0; k < q->nr; ++k) gr += q->a_r[k] * xl[1 + len - q->nr + k];
-                gl = gl + q->b_l;
-                gr = gr + q->b_r;
+                if (q->periodic == 1) {
+                    gl = xl[len];
+                    gr = xl[1];
+                } else if (q->periodic == 2) {
+                    gl = xl[1];
+                    gr = xl[len];
+                } else {
+                    for (int k = 0; k < q->nl; ++k) gl += q->a_l[k] * xl[1 + k];
+                    for (int k = 0; k < q->nr; ++k) gr += q->a_r[k] * xl[1 + len - q->nr + k];
+                    gl = gl + q->b_l;
+                    gr = gr + q->b_r;
+                }
                 xl[0] = gl;
                 xl[len + 1] = gr;
                 if (ndim == 1 && bpv_1d) orc_mul_line_bpv(op, gl, gr, xl + 1, ol, 1);

@@ -51,14 +51,19 @@ typedef struct orc_op {
     const double* coeff;
 } orc_op;
 
-/* One affine boundary condition along an axis (bc_operators.jl:21-39): the ghosts are
- * l = a_l . u[1:nl] + b_l and r = a_r . u[end-nr+1:end] + b_r. */
+/* One boundary condition along an axis.  Affine (bc_operators.jl:21-39): the ghosts are
+ * l = a_l . u[1:nl] + b_l and r = a_r . u[end-nr+1:end] + b_r.  periodic = 1 is the atomic
+ * PeriodicBC on a line, l = u[end], r = u[1] (bc_operators.jl:192); periodic = 2 is the
+ * padding of a MultiDimDirectionalBC / ComposedMultiDimBC whose BCs are all PeriodicBC,
+ * whose ghost planes are selectdim(u, D, 1) and selectdim(u, D, size(u, D)), i.e.
+ * l = u[1], r = u[end] (multi_dim_bc_operators.jl:221-252).  The taps are ignored then. */
 typedef struct orc_bc {
     int nl, nr;
     const double* a_l;
     double b_l;
     const double* a_r;
     double b_r;
+    int periodic;
 } orc_bc;
 
 /* fornberg.jl:7-62 */

@@ -34,7 +34,7 @@ class OrcOp(C.Structure):
 class OrcBC(C.Structure):
     _fields_ = [
         ("nl", C.c_int), ("nr", C.c_int), ("a_l", _dp), ("b_l", C.c_double),
-        ("a_r", _dp), ("b_r", C.c_double),
+        ("a_r", _dp), ("b_r", C.c_double), ("periodic", C.c_int),
     ]
 
 
@@ -179,16 +179,22 @@ class Op:
 class BC:
     """An affine (Robin-family) BC: ghost = a . u[1:k] + b on each side."""
 
-    def __init__(self, a_l, b_l, a_r, b_r):
+    def __init__(self, a_l, b_l, a_r, b_r, periodic=0):
         self.a_l = np.ascontiguousarray(a_l, dtype=np.float64)
         self.a_r = np.ascontiguousarray(a_r, dtype=np.float64)
         self.b_l, self.b_r = float(b_l), float(b_r)
+        self.periodic = int(periodic)   # 1: line wrap, 2: N-D periodic padding (first/last)
 
     def struct(self) -> OrcBC:
-        return OrcBC(len(self.a_l), len(self.a_r), _ptr(self.a_l), self.b_l, _ptr(self.a_r), self.b_r)
+        return OrcBC(len(self.a_l), len(self.a_r), _ptr(self.a_l), self.b_l, _ptr(self.a_r), self.b_r,
+                     int(self.periodic))
 
     def pad(self, u):
         u = np.asarray(u, dtype=np.float64)
+        if self.periodic == 1:
+            return float(u[-1]), float(u[0])
+        if self.periodic == 2:
+            return float(u[0]), float(u[-1])
         gl = float(np.dot(self.a_l, u[: len(self.a_l)]) + self.b_l)
         gr = float(np.dot(self.a_r, u[len(u) - len(self.a_r):]) + self.b_r)
         return gl, gr
@@ -219,6 +225,18 @@ def dirichlet(vl, vr) -> BC:
 
 def dirichlet0() -> BC:
     return dirichlet(0.0, 0.0)
+
+
+def periodic() -> BC:
+    """PeriodicBC(T) applied to a line: ghosts u[end], u[1] (bc_operators.jl:192)."""
+    return BC([], 0.0, [], 0.0, periodic=1)
+
+
+def periodic_nd() -> BC:
+    """PeriodicBC padding of an N-D array through MultiDimDirectionalBC{T,PeriodicBC{T}} /
+    ComposedMultiDimBC{T,PeriodicBC{T}}: ghosts u[1], u[end] (multi_dim_bc_operators.jl:
+    221-252)."""
+    return BC([], 0.0, [], 0.0, periodic=2)
 
 
 def neumann(g, dx, order=1) -> BC:

@@ -226,3 +226,59 @@ def test_ghost_accumulate_and_bad_inputs():
         D.mul_(ud, A, ud)                         # aliasing
     with pytest.raises(AssertionError):
         A * to_dev(rng.standard_normal((n, n + 1)))   # BC sized for (n, n)
+
+
+# ---------------------------------------------------------------------------------------
+# PeriodicBC: line wrap in 1-D (bc_operators.jl:192), first/last planes for an all-periodic
+# N-D BC (multi_dim_bc_operators.jl:221-252), wrap per line when composed with other BCs
+
+def obc_any(Q, ndim, jsize):
+    from diffeqoperators_amd.apply import _PeriodicPad, _bcs_of
+    bcs, _ = _bcs_of(Q, ndim, jsize)
+    out = []
+    for k in range(ndim):
+        b = bcs[k]
+        if isinstance(b, _PeriodicPad):
+            out.append(po.periodic() if b.kind == _abi.BC_PERIODIC else po.periodic_nd())
+        else:
+            out.append(obc(b))
+    return out
+
+
+def test_periodic_1d_second_order_check():
+    h = 0.025 * np.pi
+    N = int(round(2 * (np.pi / h)))
+    x = -np.pi + h * np.arange(N)
+    u0 = np.cos(x)
+    L2 = D.CenteredDifference(2, 2, h, N)
+    Q = D.PeriodicBC(float)
+    print("case: periodic 1-D second-order check", flush=True)
+    got = host64(L2 * Q * to_dev(u0))
+    M = np.diag(np.full(N, -2.0)) + np.diag(np.ones(N - 1), 1) + np.diag(np.ones(N - 1), -1)
+    M[-1, 0] = M[0, -1] = 1.0
+    assert rel(got, (M / (h * h)) @ u0) < 1e-12
+    assert rel(got, po.ghost_apply([oop(L2)], [po.periodic()], u0, bpv_1d=True)) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("form", ["all", "mixed"])
+@pytest.mark.parametrize("shape", [(1000, 1000), (96, 80, 72)])
+def test_periodic_nd(dtype, form, shape):
+    rng = np.random.default_rng(77)
+    js = tuple(reversed(shape))
+    nd = len(shape)
+    if form == "all":
+        Q = D.compose(*D.PeriodicBC(float, js))
+    else:
+        Q = D.compose(D.PeriodicBC[1](float, js), *[D.Dirichlet0BC[k](float, js) for k in range(2, nd + 1)])
+    ops = [D.CenteredDifference[N](2, 4, 0.01, js[N - 1]) for N in range(1, nd + 1)]
+    ops.append(D.UpwindDifference[1](1, 2, 0.01, js[0], -rng.standard_normal(js[0])))
+    A = ops[0]
+    for L in ops[1:]:
+        A = A + L
+    A = A * Q
+    u = to_dev(rng.uniform(-1, 1, shape), dtype)
+    print(f"case: periodic {form} {shape} {dtype}", flush=True)
+    got = host64(A * u)
+    ref = po.ghost_apply([oop(L) for L in ops], obc_any(Q, nd, js), host64(u))
+    assert rel(got, ref) < TOL[dtype]

@@ -21,10 +21,11 @@ def build(shape, K_half, dtype, seed, uniform=True, upwind=False):
     for N in range(1, nd + 1):
         n = jsize[N - 1]
         h = 1.0 / (n + 1)
-        dx = h if uniform else h * (1 + 0.5 * rng.random(n + 1))
+        dx = h if uniform in (True, "mixed") else h * (1 + 0.5 * rng.random(n + 1))
         # CenteredDifference(2, 2*K_half) has a 2*K_half+1 point interior window
         ops.append(D.CenteredDifference[N](2, 2 * K_half, dx, n))
-        if upwind:
+        if upwind or (uniform == "mixed" and N == 1):
+            # "mixed": per-row weights along x only (vector coefficients), uniform elsewhere
             ops.append(D.UpwindDifference[N](1, 3, dx, n, rng.standard_normal(n)))
     A = ops[0]
     for L in ops[1:]:
@@ -50,7 +51,7 @@ CASES_2D = [((64, 64), 1), ((257, 130), 2), ((100, 333), 3), ((4096, 64), 2), ((
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
 @pytest.mark.parametrize("shape,kh", CASES_2D)
-@pytest.mark.parametrize("uniform", [True, False])
+@pytest.mark.parametrize("uniform", [True, False, "mixed"])
 def test_tiled_2d_bitwise(shape, kh, dtype, uniform):
     A, u = build(shape, kh, dtype, seed=kh, uniform=uniform, upwind=not uniform and kh == 3)
     print(f"case: tiled2d {shape} K={2*kh+1} {dtype}", flush=True)
@@ -75,7 +76,7 @@ CASES_3D = [((32, 32, 32), 1), ((40, 48, 64), 2), ((33, 29, 70), 3), ((64, 64, 2
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
 @pytest.mark.parametrize("shape,kh", CASES_3D)
-@pytest.mark.parametrize("uniform", [True, False])
+@pytest.mark.parametrize("uniform", [True, False, "mixed"])
 def test_tiled_3d_bitwise(shape, kh, dtype, uniform):
     A, u = build(shape, kh, dtype, seed=10 + kh, uniform=uniform, upwind=not uniform and kh == 3)
     print(f"case: tiled3d {shape} K={2*kh+1} {dtype} uniform={uniform}", flush=True)

@@ -92,7 +92,7 @@ def test_constructor_errors():
     with pytest.raises(ValueError):
         D.compose(Qs[0])
     with pytest.raises(NotImplementedError):
-        D.PeriodicBC(float)
+        D.GeneralBC([1.0, 2.0, 3.0], 0.1, 2)
 
 
 def test_scaling_keeps_coeff_func():
@@ -163,6 +163,9 @@ def _oracle_op(L, axis):
 
 
 def _obc(bc):
+    from diffeqoperators_amd.apply import _PeriodicPad
+    if isinstance(bc, _PeriodicPad):
+        return po.periodic() if bc.kind == _abi.BC_PERIODIC else po.periodic_nd()
     return po.BC(bc.a_l, bc.b_l, bc.a_r, bc.b_r)
 
 
@@ -312,3 +315,53 @@ def test_hc_plan_rejects_bad_shapes(hc, rng):
     L = D.CenteredDifference[2](2, 2, 0.1, 10)
     with pytest.raises(ValueError):   # no BC on axis 2
         hc_apply(hc, [L], rng.standard_normal((10, 10)), bcs={0: D.Dirichlet0BC(float)})
+
+
+# ---------------------------------------------------------------------------------------
+# PeriodicBC: which padding each form of Q selects, and the device arithmetic
+
+def test_periodic_bc_forms():
+    from diffeqoperators_amd.apply import _bcs_of
+    P = D.PeriodicBC(float)
+    bcs, sem = _bcs_of(P, 1, (10,))
+    assert sem == _abi.SEM_BPV and bcs[0].kind == _abi.BC_PERIODIC
+    bcs, sem = _bcs_of(P, 2, (10, 6))            # BC*u on a matrix: MultiDimBC{1}
+    assert bcs[0].kind == _abi.BC_PERIODIC_ND
+    Qx, Qy = D.PeriodicBC(float, (10, 6))
+    bcs, _ = _bcs_of(D.compose(Qx, Qy), 2, (10, 6))
+    assert bcs[0].kind == bcs[1].kind == _abi.BC_PERIODIC_ND
+    bcs, _ = _bcs_of(D.PeriodicBC[2](float, (10, 6)), 2, (10, 6))
+    assert bcs[1].kind == _abi.BC_PERIODIC_ND
+    mixed = D.compose(D.PeriodicBC[1](float, (10, 6)), D.Dirichlet0BC[2](float, (10, 6)))
+    bcs, _ = _bcs_of(mixed, 2, (10, 6))
+    assert bcs[0].kind == _abi.BC_PERIODIC and isinstance(bcs[1], D.AffineBC)
+
+
+@pytest.mark.parametrize("wind,d,a", [(0, 2, 2), (0, 1, 4), (0, 4, 6), (1, 1, 2), (1, 2, 3)])
+def test_hc_1d_periodic_bpv(hc, rng, wind, d, a):
+    from diffeqoperators_amd.apply import _bcs_of
+    n = 31
+    u = rng.standard_normal(n)
+    L = D.UpwindDifference(d, a, 0.1, n, -1.0) if wind else D.CenteredDifference(d, a, 0.1, n)
+    bcs, sem = _bcs_of(D.PeriodicBC(float), 1, (n,))
+    got = hc_apply(hc, [L], u, sem=sem, bcs=bcs)
+    ref = po.ghost_apply([_oracle_op(L, 0)], [po.periodic()], u, bpv_1d=True)
+    assert rel(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("form", ["all", "mixed"])
+@pytest.mark.parametrize("shape", [(17, 23), (9, 10, 14)])
+def test_hc_nd_periodic(hc, rng, form, shape):
+    from diffeqoperators_amd.apply import _bcs_of
+    u = rng.standard_normal(shape)
+    js = tuple(reversed(shape))
+    nd = len(shape)
+    if form == "all":
+        Q = D.compose(*D.PeriodicBC(float, js))
+    else:
+        Q = D.compose(D.PeriodicBC[1](float, js), *[D.Dirichlet0BC[k](float, js) for k in range(2, nd + 1)])
+    bcs, sem = _bcs_of(Q, nd, js)
+    terms = [D.CenteredDifference[N](2, 4, 0.1, js[N - 1]) for N in range(1, nd + 1)]
+    got = hc_apply(hc, terms, u, sem=sem, bcs=bcs)
+    ref = po.ghost_apply([_oracle_op(L, L.N - 1) for L in terms], [_obc(bcs[k]) for k in range(nd)], u)
+    assert rel(got, ref) < 1e-13

@@ -249,3 +249,61 @@ def test_planted_mistake_is_caught(fixtures):
     op = po.Op(0, e["d"], e["a"], e["dx"], e["N"])
     op.high = op.low.copy()
     assert rel(op.matrix(), np.array(e["A"])) > 1e-3
+
+
+# ---------------------------------------------------------------------------------------
+# PeriodicBC
+
+def test_periodic_second_order_check():
+    # 2nd_order_check.jl:3-38: L2*Q*u0 equals the periodic tridiagonal (1,-2,1)/h^2, and
+    # L1*Q*(L1*Q*u0) the periodic (1/4, 0, -1/2, 0, 1/4)/h^2 stencil
+    ordr = 2
+    h = 0.025 * math.pi
+    N = int(round(2 * (math.pi / h)))
+    x = -math.pi + h * np.arange(N)
+    u0 = np.cos(x)
+    L1 = po.Op(0, 1, ordr, h, N)
+    L2 = po.Op(0, 2, ordr, h, N)
+    Q = po.periodic()
+    M = np.diag(np.full(N, -2.0)) + np.diag(np.ones(N - 1), 1) + np.diag(np.ones(N - 1), -1)
+    M[-1, 0] = M[0, -1] = 1.0
+    M /= h * h
+    got2 = po.ghost_apply([L2], [Q], u0, bpv_1d=True)
+    assert rel(got2, M @ u0) < 1e-12
+    A = np.zeros((N, N))
+    for i in range(N):
+        for j in range(N):
+            if i == j:
+                A[i, j] = -0.5
+            if abs(i - j) == 2:
+                A[i, j] = 0.25
+    A[-2, 0] = A[-1, 1] = A[0, -2] = A[1, -1] = 0.25
+    A /= h * h
+    t = po.ghost_apply([L1], [Q], u0, bpv_1d=True)
+    assert rel(po.ghost_apply([L1], [Q], t, bpv_1d=True), A @ u0) < 1e-12
+    assert np.max(-np.cos(x) - got2) < 0.001
+
+
+def test_periodic_upwind_bpv(fixtures):
+    # upwind_operators_interface.jl:177-566: L * (PeriodicBC * x[1:N]) == analyticL * bpv
+    rng = np.random.default_rng(4)
+    for e in fixtures["upwind_uniform"]:
+        n = len(e["A"])
+        op = po.Op(1, e["d"], e["a"], e["dx"], n, coeff=e["coeff"])
+        x = rng.random(n)
+        bpv = np.concatenate([[x[-1]], x, [x[0]]])
+        got = po.ghost_apply([op], [po.periodic()], x, bpv_1d=True)
+        assert rel(got, np.array(e["A"]) @ bpv) < 1e-13, e["name"]
+
+
+def test_periodic_nd_padding_is_edge_planes(rng):
+    # multi_dim_bc_operators.jl:221-252: an all-periodic N-D BC pads with the first and last
+    # planes of u; a line wrap is what the atomic BC gives a vector (bc_operators.jl:192)
+    u = rng.standard_normal((7, 12))   # Julia (12, 7)
+    L = po.Op(0, 2, 2, 0.1, 12, axis=0)
+    nd = po.ghost_apply([L], [po.periodic_nd(), po.periodic_nd()], u)
+    wrap = po.ghost_apply([L], [po.periodic(), po.periodic()], u)
+    for j in range(7):
+        line = u[j]
+        assert rel(nd[j], L.mul_padded(np.concatenate([[line[0]], line, [line[-1]]]))) < 1e-15
+        assert rel(wrap[j], L.mul_padded(np.concatenate([[line[-1]], line, [line[0]]]))) < 1e-15
