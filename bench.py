@@ -333,7 +333,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="laplacian2d",
-                    choices=["laplacian2d", "nonuniform3d"])
+                    choices=["laplacian2d", "nonuniform3d", "batched", "served"])
     ap.add_argument("--no-also", action="store_true",
                     help="skip the other workloads (3-D non-uniform, batched RK4, served)")
     ap.add_argument("--requests", type=int, default=200, help="served workload: requests")
@@ -369,6 +369,27 @@ def main():
                        "grid": [4096, 4096], "storage": "f64", "accumulation": "f64",
                        "tile_variant": r["variant"]},
             "roofline": r["roofline"],
+        }
+    elif args.config == "batched":
+        rb = bench_batched(max(10, args.steps // 10))
+        line = {
+            "metric": METRIC, "value": rb["rhs_gpts_s"], "unit": "Gpts/s (RHS evaluations)",
+            "n_gpus": 1, "steps": max(10, args.steps // 10), "warmup": 1,
+            "ms_per_step": rb["rk4_step_ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "1024 independent 1-D/2-D Δ*Q*u problems, one RK4 step per step"},
+            "batched": rb,
+        }
+    elif args.config == "served":
+        rs = bench_served(args.requests)
+        line = {
+            "metric": METRIC, "value": rs["served"]["solves_per_s"], "unit": "solves/s",
+            "n_gpus": 1, "steps": args.requests, "warmup": 0,
+            "ms_per_step": round(1e3 / max(rs["served"]["solves_per_s"], 1e-9), 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "Poisson-arrival 2-D advection-diffusion "
+                                                        "method-of-lines solves, 100 RK4 steps each"},
+            "served": rs,
         }
     else:
         r3 = bench_nonuniform3d(args.steps, args.warmup)

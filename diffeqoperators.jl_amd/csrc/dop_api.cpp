@@ -202,7 +202,7 @@ namespace {
 int batch_upload(dop_batch* b, int chunk)
 {
     for (auto& P : b->probs) P.chunk = chunk;
-    dop::batch_items(b->probs, &b->items);
+    dop::batch_items(b->probs, &b->items, &b->groups);
     b->chunk = chunk;
     b->nitems = (int)b->items.size();
     if (!b->d_probs) {
@@ -238,7 +238,7 @@ int batch_init(dop_batch* b, int n, const dop_plan* const* plans)
 
 hipError_t batch_launch(const dop_batch* b, int stage, int accumulate, hipStream_t s)
 {
-    return dop::launch_batch_items(b->storage, b->d_probs, b->d_items, b->nitems, stage,
+    return dop::launch_batch_items(b->storage, b->d_probs, b->d_items, b->groups, stage,
                                    accumulate, b->grid, s);
 }
 

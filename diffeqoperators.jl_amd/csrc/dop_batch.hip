@@ -3,7 +3,9 @@
 // The host turns a batch into a work list: every 2-D problem with a tiled path contributes
 // row-chunk tiles (4 wave segments x `chunk` rows) plus chunks of its shell points; every
 // other problem (1-D, 3-D, ...) contributes chunks of points for the generic evaluator.
-// Blocks walk the list with a static stride.  A 2-D tile keeps one point per lane and
+// The list is sorted into groups of one kind, window width and accumulator; each group is
+// one launch of a kernel specialised for it (registers sized for that case alone), whose
+// blocks walk the group's items with a static stride.  A 2-D tile keeps one point per lane and
 // takes the x window from neighbouring lanes with DPP wavefront shifts (wave_shr/wave_shl
 // by one lane per step), so a wave's 64 lanes yield 64-2H outputs per row; the y window
 // comes from a register ring with PF rows of prefetch, as in the single-plan kernel.
@@ -15,6 +17,7 @@
 
 #include "dop_batch.hpp"
 #include "dop_eval.hpp"
+#include "dop_ring.hpp"
 
 namespace dop {
 
@@ -126,7 +129,6 @@ __device__ void tile2d(const BProb& B, const StageBufs<T>& S, int stage, int acc
     const int n1 = (int)B.n1;
     const int64_t s1 = B.ys;
     const int64_t x = (int64_t)B.lo[0] + (int64_t)seg * OUTW + (lane - H);   // this lane's column
-    const bool xin = x >= 0 && x < n0;
     const bool out_ok = lane >= H && lane < 64 - H && x < B.hi[0];
     const W* wxt = btab<ACC>(B, 0);
     const W* wyt = btab<ACC>(B, 1);
@@ -137,17 +139,20 @@ __device__ void tile2d(const BProb& B, const StageBufs<T>& S, int stage, int acc
         for (int k = 0; k < K; ++k) wx[k] = wxt[xr * K + k];
     }
     W q[R];
+    // clamped loads on every lane: a lane outside [0, n0) holds another column's value and
+    // stores nothing; its neighbours' x windows never reach it (the box is H from the edges)
+    const int64_t xc = min(max(x, (int64_t)0), n0 - 1);
     auto load_row = [&](int row) -> W {
         row = min(max(row, 0), n1 - 1);
-        return xin ? (W)Store<T>::ld(S.X, (int64_t)row * s1 + x) : W(0);
+        return (W)Store<T>::ld(S.X, (int64_t)row * s1 + xc);
     };
 #pragma unroll
     for (int s = 0; s < R - 1; ++s) q[s] = load_row(y0 - H + s);
     for (int yb = y0; yb < y1; yb += R) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
+        auto step = [&](auto jc) -> bool {
+            constexpr int j = decltype(jc)::value;
             const int y = yb + j;
-            if (y >= y1) break;
+            if (y >= y1) return false;
             q[(j + K - 1 + PF) % R] = load_row(y + H + PF);
             // x window of the centre row from the neighbouring lanes
             W xw[K];
@@ -172,7 +177,9 @@ __device__ void tile2d(const BProb& B, const StageBufs<T>& S, int stage, int acc
 #pragma unroll
             for (int k = 0; k < K; ++k) acc.add(wyt[yr * K + k], q[(j + k) % R]);
             if (out_ok) epilogue<T>(B, S, stage, accumulate, (int64_t)y * s1 + x, acc.value());
-        }
+            return true;
+        };
+        if (!ring_steps<0, R>(step)) break;
     }
 }
 
@@ -202,67 +209,88 @@ __device__ void shell_chunk(const BProb& B, const StageBufs<T>& S, int stage, in
     }
 }
 
-template <typename T, int ACC>
-__device__ void run_item(const BProb& B, const BItem& I, int stage, int accumulate)
-{
-    const StageBufs<T> S = stage_bufs<T>(B, stage);
-    if (I.kind == 1) {
-        shell_chunk<T, ACC>(B, S, stage, accumulate, I.a);
-        return;
-    }
-    switch (B.K) {
-    case 3: tile2d<T, ACC, 3>(B, S, stage, accumulate, I.a, I.b); break;
-    case 5: tile2d<T, ACC, 5>(B, S, stage, accumulate, I.a, I.b); break;
-    case 7: tile2d<T, ACC, 7>(B, S, stage, accumulate, I.a, I.b); break;
-    case 9: tile2d<T, ACC, 9>(B, S, stage, accumulate, I.a, I.b); break;
-    case 11: tile2d<T, ACC, 11>(B, S, stage, accumulate, I.a, I.b); break;
-    case 13: tile2d<T, ACC, 13>(B, S, stage, accumulate, I.a, I.b); break;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_batch_items(const BProb* __restrict__ probs,
-                                                      const BItem* __restrict__ items,
-                                                      int nitems, int stage, int accumulate)
+// one group of 2-D tiles with window K and accumulator ACC
+template <typename T, int ACC, int K>
+__global__ __launch_bounds__(kBlock) void k_batch_tiles(const BProb* __restrict__ probs,
+                                                      const BItem* __restrict__ items, int nitems,
+                                                      int stage, int accumulate)
 {
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const BItem I = items[it];
         const BProb& B = probs[I.prob];
-        const int acc = B.plan_accum;
-        if constexpr (sizeof(T) == 8) {
-            run_item<T, kAccF64>(B, I, stage, accumulate);
-        } else {
-            if (acc == kAccF32Kahan) run_item<T, kAccF32Kahan>(B, I, stage, accumulate);
-            else if (acc == kAccF64) run_item<T, kAccF64>(B, I, stage, accumulate);
-            else run_item<T, kAccF32>(B, I, stage, accumulate);
-        }
+        tile2d<T, ACC, K>(B, stage_bufs<T>(B, stage), stage, accumulate, I.a, I.b);
+    }
+}
+
+// one group of generic point chunks with accumulator ACC
+template <typename T, int ACC>
+__global__ __launch_bounds__(kBlock) void k_batch_points(const BProb* __restrict__ probs,
+                                                       const BItem* __restrict__ items, int nitems,
+                                                       int stage, int accumulate)
+{
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BItem I = items[it];
+        const BProb& B = probs[I.prob];
+        shell_chunk<T, ACC>(B, stage_bufs<T>(B, stage), stage, accumulate, I.a);
+    }
+}
+
+template <typename T, int ACC>
+hipError_t launch_group(const BGroup& g, const BProb* probs, const BItem* items, int stage,
+                        int accumulate, int grid, hipStream_t s)
+{
+    const dim3 gr((unsigned)(g.count < grid ? g.count : grid));
+    const BItem* it = items + g.first;
+    if (g.kind == 1) {
+        hipLaunchKernelGGL((k_batch_points<T, ACC>), gr, dim3(kBlock), 0, s, probs, it, g.count,
+                           stage, accumulate);
+        return hipGetLastError();
+    }
+    switch (g.K) {
+#define DOP_BK(KK)                                                                               \
+    case KK:                                                                                      \
+        hipLaunchKernelGGL((k_batch_tiles<T, ACC, KK>), gr, dim3(kBlock), 0, s, probs, it,         \
+                           g.count, stage, accumulate);                                           \
+        return hipGetLastError();
+        DOP_BK(3) DOP_BK(5) DOP_BK(7) DOP_BK(9) DOP_BK(11) DOP_BK(13)
+#undef DOP_BK
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_group_any(const BGroup& g, const BProb* probs, const BItem* items, int stage,
+                            int accumulate, int grid, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 8) {
+        if (g.acc != kAccF64) return hipErrorInvalidValue;
+        return launch_group<T, kAccF64>(g, probs, items, stage, accumulate, grid, s);
+    } else {
+        if (g.acc == kAccF32Kahan)
+            return launch_group<T, kAccF32Kahan>(g, probs, items, stage, accumulate, grid, s);
+        if (g.acc == kAccF64) return launch_group<T, kAccF64>(g, probs, items, stage, accumulate, grid, s);
+        return launch_group<T, kAccF32>(g, probs, items, stage, accumulate, grid, s);
     }
 }
 
 }  // namespace
 
-hipError_t launch_batch_items(int storage, const BProb* probs, const BItem* items, int nitems,
-                              int stage, int accumulate, int grid, hipStream_t s)
+hipError_t launch_batch_items(int storage, const BProb* probs, const BItem* items,
+                              const std::vector<BGroup>& groups, int stage, int accumulate,
+                              int grid, hipStream_t s)
 {
-    if (nitems < 1) return hipSuccess;
-    const unsigned g = (unsigned)(nitems < grid ? nitems : grid);
-    switch (storage) {
-    case kF64:
-        hipLaunchKernelGGL(k_batch_items<double>, dim3(g), dim3(kBlock), 0, s, probs, items, nitems,
-                           stage, accumulate);
-        break;
-    case kF32:
-        hipLaunchKernelGGL(k_batch_items<float>, dim3(g), dim3(kBlock), 0, s, probs, items, nitems,
-                           stage, accumulate);
-        break;
-    case kBF16:
-        hipLaunchKernelGGL(k_batch_items<bf16>, dim3(g), dim3(kBlock), 0, s, probs, items, nitems,
-                           stage, accumulate);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    for (const BGroup& g : groups) {
+        if (g.count < 1) continue;
+        hipError_t e;
+        switch (storage) {
+        case kF64: e = launch_group_any<double>(g, probs, items, stage, accumulate, grid, s); break;
+        case kF32: e = launch_group_any<float>(g, probs, items, stage, accumulate, grid, s); break;
+        case kBF16: e = launch_group_any<bf16>(g, probs, items, stage, accumulate, grid, s); break;
+        default: e = hipErrorInvalidValue;
+        }
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 }  // namespace dop

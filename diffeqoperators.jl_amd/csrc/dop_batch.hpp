@@ -37,12 +37,21 @@ struct BItem {
     int32_t prob, kind, a, b;
 };
 
-// Fills probs[i] geometry from plan i and appends its work items for row chunk `chunk`.
-void batch_geometry(const DevicePlan& p, int chunk, BProb* B);
-void batch_items(const std::vector<BProb>& probs, std::vector<BItem>* items);
+// A run of items of one kind (0: 2-D tile, 1: generic points), window width K (tiles) and
+// accumulator; one launch each.
+struct BGroup {
+    int32_t kind, K, acc, first, count;
+};
 
-// stage < 0: plain apply; 0..3: RK4 stage
-hipError_t launch_batch_items(int storage, const BProb* probs, const BItem* items, int nitems,
-                              int stage, int accumulate, int grid, hipStream_t s);
+// Fills probs[i] geometry from plan i.
+void batch_geometry(const DevicePlan& p, int chunk, BProb* B);
+// Work items of all problems for row chunk B.chunk, sorted into groups.
+void batch_items(const std::vector<BProb>& probs, std::vector<BItem>* items,
+                 std::vector<BGroup>* groups);
+
+// stage < 0: plain apply; 0..3: RK4 stage.  One launch per group, in order, on stream s.
+hipError_t launch_batch_items(int storage, const BProb* probs, const BItem* items,
+                              const std::vector<BGroup>& groups, int stage, int accumulate,
+                              int grid, hipStream_t s);
 
 }  // namespace dop

@@ -1,6 +1,7 @@
 // dop_batch_host.cpp -- host side of the persistent batched launch: per-problem tile
 // geometry and the work list.
 #include <algorithm>
+#include <tuple>
 
 #include "dop_batch.hpp"
 
@@ -49,8 +50,14 @@ void batch_geometry(const DevicePlan& p, int chunk, BProb* B)
     }
 }
 
-void batch_items(const std::vector<BProb>& probs, std::vector<BItem>* items)
+void batch_items(const std::vector<BProb>& probs, std::vector<BItem>* items,
+                 std::vector<BGroup>* groups)
 {
+    // group key: kind, window, accumulator
+    auto key = [&](const BItem& I) {
+        const BProb& B = probs[I.prob];
+        return std::make_tuple(I.kind, I.kind == 0 ? B.K : 0, B.plan_accum);
+    };
     items->clear();
     for (size_t i = 0; i < probs.size(); ++i) {
         const BProb& B = probs[i];
@@ -61,6 +68,17 @@ void batch_items(const std::vector<BProb>& probs, std::vector<BItem>* items)
         }
         const int64_t ns = ceil_div(B.shell.start[B.shell.nbox], kChunkPts);
         for (int64_t c = 0; c < ns; ++c) items->push_back({(int32_t)i, 1, (int32_t)c, 0});
+    }
+    std::stable_sort(items->begin(), items->end(),
+                     [&](const BItem& a, const BItem& b) { return key(a) < key(b); });
+    groups->clear();
+    for (size_t k = 0; k < items->size(); ++k) {
+        const BItem& I = (*items)[k];
+        if (groups->empty() || key((*items)[groups->back().first]) != key(I)) {
+            const auto t = key(I);
+            groups->push_back({std::get<0>(t), std::get<1>(t), std::get<2>(t), (int32_t)k, 0});
+        }
+        ++groups->back().count;
     }
 }
 

@@ -49,11 +49,12 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     // points per lane: 16-byte lane loads when rows stay aligned
     const int vx_native = (P.storage == kF64) ? 2 : 4;
     fp->VX = (P.n[0] % vx_native == 0) ? vx_native : 1;
-    if (P.ndim == 3) {
-        // 3-D lanes store whole VX groups: the box's x edges are VX-aligned (the few columns
-        // this moves into the shell are evaluated there)
-        A.lo[0] = (int)ceil_div(A.lo[0], fp->VX) * fp->VX;
-        A.hi[0] = A.hi[0] / fp->VX * fp->VX;
+    {
+        // lanes store whole VX groups: the box's x edges are aligned to the widest VX of the
+        // tile variants (the few columns this moves into the shell are evaluated there)
+        const int al = (P.ndim == 2 && fp->VX > 1 && P.n[0] % (2 * fp->VX) == 0) ? 2 * fp->VX : fp->VX;
+        A.lo[0] = (int)ceil_div(A.lo[0], al) * al;
+        A.hi[0] = A.hi[0] / al * al;
         if (A.hi[0] <= A.lo[0]) return;
     }
     // padded weights (rows outside the box are zero).  The tiled kernels index every axis
@@ -138,10 +139,7 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         if (2 * (A.K / 2) >= 8) A.W = 16;   // an 8-wave block must compute at least two rows
         A.VX = ((v >> 3) & 1) ? fp.VX / 2 : fp.VX;
     }
-    if (fp.kind == 2) {
-        A.xbase = (A.lo[0] / A.VX) * A.VX;
-        A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, 64 * A.VX);
-    } else {
+    {
         // segments of 64*VX points overlapping by 2*PADL (the x halo, whole lanes)
         const int H = A.K / 2;
         const int padl = (H + A.VX - 1) / A.VX * A.VX;

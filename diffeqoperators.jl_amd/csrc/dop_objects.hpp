@@ -32,6 +32,7 @@ struct dop_plan {
 struct dop_batch {
     std::vector<dop::BProb> probs;
     std::vector<dop::BItem> items;
+    std::vector<dop::BGroup> groups;
     dop::BProb* d_probs = nullptr;
     dop::BItem* d_items = nullptr;
     int nitems = 0;

@@ -106,6 +106,7 @@ struct TableSlot {
     dop::BProb* d_probs = nullptr;
     dop::BItem* d_items = nullptr;
     size_t cap_probs = 0, cap_items = 0;
+    std::vector<dop::BGroup> groups;  // launch ranges of the work list in this slot
     hipEvent_t used = nullptr;       // last step that read this slot
     bool armed = false;
 };
@@ -252,10 +253,11 @@ int dop_server_run(dop_server* srv)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
             return dop::set_error(DOP_EHIP, "server: event creation failed");
     size_t next_rel = 0, ndone = 0, ev_i = 0;
-    int slot = 0, cur_slot = -1, nitems = 0;
+    int slot = 0, cur_slot = -1;
     bool dirty = true;
     std::vector<dop::BProb> probs;
     std::vector<dop::BItem> items;
+    std::vector<dop::BGroup> groups;
     const auto t0 = Clock::now();
     auto now = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
     int rc = DOP_OK;
@@ -306,7 +308,7 @@ int dop_server_run(dop_server* srv)
                     B.scale = r.scale;
                     B.dt = r.dt;
                 }
-                dop::batch_items(probs, &items);
+                dop::batch_items(probs, &items, &groups);
                 if ((rc = ensure_slot(S, probs.size(), items.size())) != DOP_OK) break;
                 std::copy(probs.begin(), probs.end(), S.h_probs);
                 std::copy(items.begin(), items.end(), S.h_items);
@@ -318,12 +320,12 @@ int dop_server_run(dop_server* srv)
                     break;
                 }
                 cur_slot = slot;
-                nitems = (int)items.size();
+                S.groups = groups;
                 dirty = false;
             }
             TableSlot& S = srv->slots[cur_slot];
             for (int stage = 0; stage < 4 && rc == DOP_OK; ++stage)
-                if (dop::launch_batch_items(srv->storage, S.d_probs, S.d_items, nitems, stage, 0,
+                if (dop::launch_batch_items(srv->storage, S.d_probs, S.d_items, S.groups, stage, 0,
                                             srv->grid, st) != hipSuccess)
                     rc = dop::set_error(DOP_EHIP, "server: stage launch failed");
             if (rc != DOP_OK) break;

@@ -1,10 +1,10 @@
 // dop_tiled_impl.hpp -- tiled interior kernels for 2-D / 3-D Q*u plans, instantiated per
 // storage type by dop_tiled_f64.hip / dop_tiled_f32.hip / dop_tiled_bf16.hip.
 //
-// 2-D: a wave owns a 64*VX-point x segment and marches a chunk of rows.  The K rows of the
-//      y stencil live in a register ring of K+PF rows (PF rows prefetched ahead), so every
-//      row of u is loaded once per chunk; the x window of the centre row comes from lane
-//      loads that hit L1 (the row was loaded by the same wave PF+H steps earlier).
+// 2-D: a wave loads an aligned 64*VX-point row segment and marches a chunk of rows.  The K
+//      rows of the y stencil live in a register ring of K+PF rows (PF rows prefetched
+//      ahead), so every row of u is loaded once per chunk; the x window of the centre row
+//      comes from neighbouring lanes (DPP), segments overlapping by the x halo.
 // 3-D: a block of W waves owns W consecutive y rows of one x segment and marches a chunk
 //      of z planes with the same register ring along z.  Each step stages the centre plane
 //      (own rows + 2H halo rows, each with H x-halo columns) in LDS, double-buffered, one
@@ -25,6 +25,7 @@
 
 #include "dop_eval.hpp"
 #include "dop_kernels.hpp"
+#include "dop_ring.hpp"
 
 namespace dop {
 namespace tiled {
@@ -218,70 +219,6 @@ __device__ inline double dpp_shl1(double v)
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// x window of a lane's VX consecutive points: xw[e] = u(xs - H + e), e < VX + 2H.  The
-// centre comes from the lane's own registers, the sides from the neighbouring lanes through
-// DPP shifts (all 64 lanes active); only the DL edge lanes of the wave load what lies
-// outside its segment.
-template <typename T, int VX, int H>
-__device__ inline void x_window(const typename Store<T>::C* own, typename Store<T>::C* xw, int lane,
-                                const T* __restrict__ prow, int64_t xs, int64_t n0)
-{
-    using C = typename Store<T>::C;
-    constexpr int DL = (H + VX - 1) / VX;   // lanes reached on each side
-#pragma unroll
-    for (int v = 0; v < VX; ++v) xw[H + v] = own[v];
-    C l[VX], r[VX];
-#pragma unroll
-    for (int v = 0; v < VX; ++v) {
-        l[v] = own[v];
-        r[v] = own[v];
-    }
-#pragma unroll
-    for (int d = 1; d <= DL; ++d) {
-#pragma unroll
-        for (int v = 0; v < VX; ++v) {
-            l[v] = dpp_shr1(l[v]);   // now holds lane (lane - d)'s value v
-            r[v] = dpp_shl1(r[v]);   // lane (lane + d)'s value v
-        }
-#pragma unroll
-        for (int v = 0; v < VX; ++v) {
-            const int el = H - d * VX + v;          // u(xs - d*VX + v)
-            if (el >= 0 && el < H) xw[el] = l[v];
-            const int er = H + VX + (d - 1) * VX + v;   // u(xs + d*VX + v)
-            if (er < VX + 2 * H) xw[er] = r[v];
-        }
-    }
-    if (lane < DL) {
-#pragma unroll
-        for (int e = 0; e < H; ++e) {
-            const int64_t x = xs - H + e;
-            if (x >= xs - lane * VX - H && x < xs - lane * VX)   // outside this wave's segment
-                xw[e] = (x >= 0 && x < n0) ? (C)Store<T>::ld(prow, x) : C(0);
-        }
-    }
-    if (lane >= 64 - DL) {
-#pragma unroll
-        for (int e = 0; e < H; ++e) {
-            const int64_t x = xs + VX + e;
-            if (x >= xs + (64 - lane) * VX)   // beyond the wave's segment
-                xw[H + VX + e] = (x >= 0 && x < n0) ? (C)Store<T>::ld(prow, x) : C(0);
-        }
-    }
-}
-
-// Calls f(std::integral_constant<int, J>) for J = J0 .. R-1 until f returns false: the
-// steps of a register ring with compile-time slot indices, whatever the body's size.
-template <int J, int R, typename F>
-__device__ __forceinline__ bool ring_steps(F& f)
-{
-    if constexpr (J == R) {
-        return true;
-    } else {
-        if (!f(std::integral_constant<int, J>{})) return false;
-        return ring_steps<J + 1, R>(f);
-    }
-}
-
 // logical tile of this block: each XCD (blocks b, b+8, b+16, ...) gets a contiguous band
 __device__ inline int tile_of(int b, int nb, int remap)
 {
@@ -321,7 +258,46 @@ __device__ inline void shell_points(const PlanDev& P, const ShellArgs& S, T* __r
 
 constexpr int kShellPtsPerThread = 4;
 
+// x window of a lane's VX consecutive points from its own registers and its neighbours'
+// through DPP shifts: xw[e] = u(xs - H + e), e < VX + 2H.  Lanes within ceil(H/VX) of the
+// wave's ends get zeros for what lies beyond the wave; those lanes only load.
+template <typename T, int VX, int H>
+__device__ inline void x_window_dpp(const typename Store<T>::C* own, typename Store<T>::C* xw)
+{
+    using C = typename Store<T>::C;
+    constexpr int DL = (H + VX - 1) / VX;   // lanes reached on each side
+#pragma unroll
+    for (int v = 0; v < VX; ++v) xw[H + v] = own[v];
+    C l[VX], r[VX];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+        l[v] = own[v];
+        r[v] = own[v];
+    }
+#pragma unroll
+    for (int d = 1; d <= DL; ++d) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+            l[v] = dpp_shr1(l[v]);   // now holds lane (lane - d)'s value v
+            r[v] = dpp_shl1(r[v]);   // lane (lane + d)'s value v
+        }
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+            const int el = H - d * VX + v;              // u(xs - d*VX + v)
+            if (el >= 0 && el < H) xw[el] = l[v];
+            const int er = H + VX + (d - 1) * VX + v;   // u(xs + d*VX + v)
+            if (er < VX + 2 * H) xw[er] = r[v];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
+// 2-D: a wave loads an aligned 64*VX-point row segment and marches a chunk of rows with a
+// register ring of K + PF rows (PF rows prefetched ahead).  The segment's first and last
+// PADL points (whole lanes) are the x halo: segments overlap by 2*PADL, the x window comes
+// from neighbouring lanes through DPP, and only the middle lanes store.  Loads are clamped
+// into the array and issued on every path, stores are one predicated vector store, so the
+// in-order load counter waits stay precise and the ring stays in flight.
 template <typename T, int VX, int K, bool UNI, int PF>
 __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev* __restrict__ Pp,
                                                 const ShellArgs SH, int nshell,
@@ -332,6 +308,8 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     constexpr int H = K / 2;
     constexpr int R = K + PF;
     constexpr int NWIN = VX + 2 * H;
+    constexpr int PADL = (H + VX - 1) / VX * VX;   // x halo, whole lanes
+    constexpr int LEDGE = PADL / VX;               // load-only lanes at each end
     if ((int)blockIdx.x < nshell) {
         const int64_t per = 256 * kShellPtsPerThread;
         shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
@@ -349,9 +327,11 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     const int64_t n0 = A.n[0];
     const int n1 = (int)A.n[1];
     const int64_t s1 = A.s1;
-    const int64_t xs = (int64_t)A.xbase + (int64_t)xseg * (64 * VX) + lane * VX;
-    const bool rowok = xs + VX <= n0;
-    const bool full = xs >= A.lo[0] && xs + VX <= A.hi[0];
+    const int64_t xs = (int64_t)A.xbase + (int64_t)xseg * (64 * VX - 2 * PADL) + lane * VX;
+    // A lane whose group lies outside [0, n0) loads a clamped (other) group: only outputs in
+    // the box are stored and their windows stay within H of the box, inside the array.
+    const int64_t xl = min(max(xs, (int64_t)0), n0 - VX);
+    const bool st_ok = lane >= LEDGE && lane < 64 - LEDGE && xs >= A.lo[0] && xs < A.hi[0];
 
     const C* wxt = wtab<C>(A, 0);
     const C* wyt = wtab<C>(A, 1);
@@ -369,28 +349,23 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
         for (int k = 0; k < K; ++k) wyu[k] = wyt[k];
     }
 
+    const T* ucol = u + xl;
     C q[R][VX];
     auto load_row = [&](int row, C* dst) {
         row = min(max(row, 0), n1 - 1);
-        const T* p = u + (int64_t)row * s1 + xs;
-        if (rowok) {
-            VecIO<T, VX>::ld(p, dst);
-        } else {
-#pragma unroll
-            for (int v = 0; v < VX; ++v) dst[v] = (xs + v < n0) ? (C)Store<T>::ld(p, v) : C(0);
-        }
+        VecIO<T, VX>::ld(ucol + (int64_t)row * s1, dst);
     };
 #pragma unroll
     for (int s = 0; s < R - 1; ++s) load_row(y0 - H + s, q[s]);
 
     for (int yb = y0; yb < y1; yb += R) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
+        auto step = [&](auto jc) -> bool {
+            constexpr int j = decltype(jc)::value;
             const int y = yb + j;
-            if (y >= y1) break;
+            if (y >= y1) return false;
             load_row(y + H + PF, q[(j + K - 1 + PF) % R]);
             C xw[NWIN];
-            x_window<T, VX, H>(q[(j + H) % R], xw, lane, u + (int64_t)y * s1, xs, n0);
+            x_window_dpp<T, VX, H>(q[(j + H) % R], xw);
             C wy[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) wy[k] = UNI ? wyu[k] : wyt[(int64_t)y * K + k];
@@ -404,21 +379,19 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
                 for (int k = 0; k < K; ++k) s = fmac<C>(wy[k], q[(j + k) % R][v], s);
                 out[v] = s;
             }
-            T* po = du + (int64_t)y * s1 + xs;
-            if (full && !accumulate) {
-                VecIO<T, VX>::st(po, out);
-            } else {
+            if (st_ok) {
+                T* po = du + (int64_t)y * s1 + xs;
+                if (accumulate) {
+                    C old[VX];
+                    VecIO<T, VX>::ld(po, old);
 #pragma unroll
-                for (int v = 0; v < VX; ++v) {
-                    const int64_t x = xs + v;
-                    if (x >= A.lo[0] && x < A.hi[0]) {
-                        double r = (double)out[v];
-                        if (accumulate) r += (double)Store<T>::ld(po, v);
-                        Store<T>::st(po, v, r);
-                    }
+                    for (int v = 0; v < VX; ++v) out[v] = (C)((double)out[v] + (double)old[v]);
                 }
+                VecIO<T, VX>::st(po, out);
             }
-        }
+            return true;
+        };
+        if (!ring_steps<0, R>(step)) break;
     }
 }
 
