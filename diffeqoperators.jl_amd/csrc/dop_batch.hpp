@@ -14,7 +14,7 @@ namespace dop {
 struct BProb {
     const PlanDev* plan;
     int32_t plan_accum;
-    int32_t tiled;        // 2-D DPP tiles cover the interior box
+    int32_t tiled;        // 1: 2-D DPP tiles cover the interior box, 2: 1-D DPP tiles
     const void* src;      // apply: u
     void* out;            // apply: du
     void* u;              // RK4: state and stage buffers
@@ -32,12 +32,15 @@ struct BProb {
     ShellArgs shell;      // points for the generic evaluator (the whole domain if !tiled)
 };
 
-// kind 0: tile (a = block column, b = row chunk); kind 1: generic chunk a of the shell
+constexpr int kMaxBatchTileK = 11;
+
+// kind 0: 2-D tile (a = block column, b = row chunk); kind 1: generic chunk a of the shell;
+// kind 2: 1-D tile (a = block column)
 struct BItem {
     int32_t prob, kind, a, b;
 };
 
-// A run of items of one kind (0: 2-D tile, 1: generic points), window width K (tiles) and
+// A run of items of one kind (0: 2-D tile, 1: generic points, 2: 1-D tile), window width K (tiles) and
 // accumulator; one launch each.
 struct BGroup {
     int32_t kind, K, acc, first, count;

@@ -97,10 +97,12 @@ int build_host_plan(const dop_desc& D, HostPlan* out, std::string* err)
     int acc = D.accum;
     if (acc == DOP_ACC_AUTO) {
         if (D.storage == DOP_F64) acc = DOP_ACC_F64;
-        // high derivative orders cancel large alternating weights: compensated f32 sum
-        else acc = (H.max_derivative_order >= 3) ? DOP_ACC_F32_KAHAN : DOP_ACC_F32;
+        // high derivative orders cancel large alternating weights: exact f32 products
+        // summed in f64
+        else acc = (H.max_derivative_order >= 3) ? DOP_ACC_F32_WIDE : DOP_ACC_F32;
     }
-    if (acc != DOP_ACC_F64 && acc != DOP_ACC_F32 && acc != DOP_ACC_F32_KAHAN) {
+    if (acc != DOP_ACC_F64 && acc != DOP_ACC_F32 && acc != DOP_ACC_F32_KAHAN &&
+        acc != DOP_ACC_F32_WIDE) {
         *err = "unknown accumulation mode";
         return DOP_EINVAL;
     }

@@ -115,6 +115,18 @@ struct Acc<kAccF32Kahan> {
     DOP_HD static const float* table(const AxisDev& X) { return X.w32; }
 };
 
+// f32 weights and values, f64 sum: every product of two f32 numbers is exact in f64, so the
+// only roundings are the f64 additions (error ~K * 2^-53 of the sum of |terms|).
+template <>
+struct Acc<kAccF32Wide> {
+    using W = float;
+    double s = 0.0;
+    DOP_HD void add(float w, float x) { s = fma((double)w, (double)x, s); }
+    DOP_HD void add_sum(double v) { s += v; }
+    DOP_HD double value() const { return s; }
+    DOP_HD static const float* table(const AxisDev& X) { return X.w32; }
+};
+
 // ---------------------------------------------------------------------------------------
 // ghost values of one line: a . u[0:nl] + b (bc_operators.jl:188-191)
 template <typename T>

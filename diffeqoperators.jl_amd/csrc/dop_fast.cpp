@@ -13,82 +13,19 @@ constexpr int kMaxK3D = 9;   // 3-D stages 2*H halo rows through 8 waves: H <= 4
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-}  // namespace
-
-void build_fast(const HostPlan& H, FastPlan* fp)
+int64_t shell_boxes(const PlanDev& P, const int32_t* lo, const int32_t* hi, ShellArgs* out)
 {
-    *fp = FastPlan{};
-    const PlanDev& P = H.meta;
-    if (!P.ghost || (P.ndim != 2 && P.ndim != 3)) return;
-    const bool f64 = P.storage == kF64 && P.accum == kAccF64;
-    const bool f32 = (P.storage == kF32 || P.storage == kBF16) &&
-                     (P.accum == kAccF32 || P.accum == kAccF32Kahan);
-    if (!f64 && !f32) return;
-    fp->kahan = P.accum == kAccF32Kahan ? 1 : 0;
-    // one symmetric window for all axes
-    int half = 0;
-    for (int a = 0; a < P.ndim; ++a) {
-        const AxisDev& X = P.ax[a];
-        if (!X.active || X.rhi <= X.rlo || X.K < 1) return;
-        half = std::max(half, std::max(-X.dmin, X.dmin + X.K - 1));
-    }
-    const int K = 2 * half + 1;
-    if (K > (P.ndim == 2 ? kMaxK2D : kMaxK3D)) return;
-    FastArgs& A = fp->args;
-    for (int a = 0; a < 3; ++a) A.n[a] = P.n[a];
-    A.s1 = P.sdu[1];
-    A.s2 = P.sdu[2];
-    A.K = K;
-    for (int a = 0; a < 3; ++a) {
-        if (a >= P.ndim) { A.lo[a] = 0; A.hi[a] = 1; continue; }
-        const AxisDev& X = P.ax[a];
-        A.lo[a] = std::max(X.rlo, half);
-        A.hi[a] = std::min(X.rhi, (int)P.n[a] - half);
-        if (A.hi[a] <= A.lo[a]) return;
-    }
-    // points per lane: 16-byte lane loads when rows stay aligned
-    const int vx_native = (P.storage == kF64) ? 2 : 4;
-    fp->VX = (P.n[0] % vx_native == 0) ? vx_native : 1;
-    {
-        // lanes store whole VX groups: the box's x edges are aligned to the widest VX of the
-        // tile variants (the few columns this moves into the shell are evaluated there)
-        const int al = (P.ndim == 2 && fp->VX > 1 && P.n[0] % (2 * fp->VX) == 0) ? 2 * fp->VX : fp->VX;
-        A.lo[0] = (int)ceil_div(A.lo[0], al) * al;
-        A.hi[0] = A.hi[0] / al * al;
-        if (A.hi[0] <= A.lo[0]) return;
-    }
-    // padded weights (rows outside the box are zero).  The tiled kernels index every axis
-    // per row unless all axes are uniform, so a uniform axis of a mixed plan is expanded to
-    // one row per point.
-    bool all_uniform = true;
-    for (int a = 0; a < P.ndim; ++a) all_uniform = all_uniform && P.ax[a].uniform;
-    for (int a = 0; a < P.ndim; ++a) {
-        const AxisDev& X = P.ax[a];
-        const AxisTables& T = H.axis[a];
-        const int rows = all_uniform ? 1 : (int)P.n[a];
-        fp->w64[a].assign((size_t)rows * K, 0.0);
-        for (int r = 0; r < rows; ++r) {
-            const int src = X.uniform ? 0 : r;
-            if (!X.uniform && (r < A.lo[a] || r >= A.hi[a])) continue;
-            for (int k = 0; k < X.K; ++k)
-                fp->w64[a][(size_t)r * K + (X.dmin + k + half)] = T.w64[(size_t)src * X.K + k];
-        }
-        fp->w32[a].resize(fp->w64[a].size());
-        for (size_t i = 0; i < fp->w64[a].size(); ++i) fp->w32[a][i] = (float)fp->w64[a][i];
-        A.uniform[a] = X.uniform;
-    }
-    // shell: the domain minus the box, slabs from the slowest axis down
-    ShellArgs& S = fp->shell;
-    S.nbox = 0;
+    ShellArgs& S = *out;
+    S = ShellArgs{};
     int64_t total = 0;
     for (int a = P.ndim - 1; a >= 0; --a) {
         for (int side = 0; side < 2; ++side) {
             int64_t org[3], ext[3];
             for (int b = 0; b < 3; ++b) {
-                if (b > a) { org[b] = A.lo[b]; ext[b] = A.hi[b] - A.lo[b]; }   // inside box
-                else if (b < a) { org[b] = 0; ext[b] = P.n[b]; }               // full
-                else if (side == 0) { org[b] = 0; ext[b] = A.lo[b]; }
-                else { org[b] = A.hi[b]; ext[b] = P.n[b] - A.hi[b]; }
+                if (b > a) { org[b] = lo[b]; ext[b] = hi[b] - lo[b]; }      // inside box
+                else if (b < a) { org[b] = 0; ext[b] = P.n[b]; }            // full
+                else if (side == 0) { org[b] = 0; ext[b] = lo[b]; }
+                else { org[b] = hi[b]; ext[b] = P.n[b] - hi[b]; }
             }
             const int64_t cnt = ext[0] * ext[1] * ext[2];
             if (cnt <= 0) continue;
@@ -102,7 +39,81 @@ void build_fast(const HostPlan& H, FastPlan* fp)
         }
     }
     S.start[S.nbox] = total;
-    fp->shell_total = total;
+    return total;
+}
+
+}  // namespace
+
+void build_fast(const HostPlan& H, FastPlan* fp)
+{
+    *fp = FastPlan{};
+    const PlanDev& P = H.meta;
+    if (!P.ghost || P.ndim < 1 || P.ndim > 3) return;
+    const bool f64 = P.storage == kF64 && P.accum == kAccF64;
+    const bool f32 = (P.storage == kF32 || P.storage == kBF16) &&
+                     (P.accum == kAccF32 || P.accum == kAccF32Kahan || P.accum == kAccF32Wide);
+    if (!f64 && !f32) return;
+    // compensated modes: batched tiles only (single plans take the generic kernel)
+    fp->kahan = (P.accum == kAccF32Kahan || P.accum == kAccF32Wide) ? 1 : 0;
+    // one symmetric window for all axes
+    int half = 0;
+    for (int a = 0; a < P.ndim; ++a) {
+        const AxisDev& X = P.ax[a];
+        if (!X.active || X.rhi <= X.rlo || X.K < 1) return;
+        half = std::max(half, std::max(-X.dmin, X.dmin + X.K - 1));
+    }
+    const int K = 2 * half + 1;
+    if (K > (P.ndim <= 2 ? kMaxK2D : kMaxK3D)) return;
+    FastArgs& A = fp->args;
+    for (int a = 0; a < 3; ++a) A.n[a] = P.n[a];
+    A.s1 = P.sdu[1];
+    A.s2 = P.sdu[2];
+    A.K = K;
+    for (int a = 0; a < 3; ++a) {
+        if (a >= P.ndim) { A.lo[a] = 0; A.hi[a] = 1; continue; }
+        const AxisDev& X = P.ax[a];
+        A.lo[a] = std::max(X.rlo, half);
+        A.hi[a] = std::min(X.rhi, (int)P.n[a] - half);
+        if (A.hi[a] <= A.lo[a]) return;
+    }
+    for (int a = 0; a < 3; ++a) {
+        fp->nat_lo[a] = A.lo[a];
+        fp->nat_hi[a] = A.hi[a];
+    }
+    // points per lane: 16-byte lane loads when rows stay aligned
+    const int vx_native = (P.storage == kF64) ? 2 : 4;
+    fp->VX = (P.n[0] % vx_native == 0) ? vx_native : 1;
+    if (P.ndim >= 2) {
+        // lanes store whole VX groups: the box's x edges are aligned to the widest VX of the
+        // tile variants (the few columns this moves into the shell are evaluated there)
+        const int al = (P.ndim == 2 && fp->VX > 1 && P.n[0] % (2 * fp->VX) == 0) ? 2 * fp->VX : fp->VX;
+        A.lo[0] = (int)ceil_div(A.lo[0], al) * al;
+        A.hi[0] = A.hi[0] / al * al;
+        if (A.hi[0] <= A.lo[0]) return;
+    }
+    // padded weights (rows outside the natural box are zero).  The tiled kernels index every axis
+    // per row unless all axes are uniform, so a uniform axis of a mixed plan is expanded to
+    // one row per point.
+    bool all_uniform = true;
+    for (int a = 0; a < P.ndim; ++a) all_uniform = all_uniform && P.ax[a].uniform;
+    for (int a = 0; a < P.ndim; ++a) {
+        const AxisDev& X = P.ax[a];
+        const AxisTables& T = H.axis[a];
+        const int rows = all_uniform ? 1 : (int)P.n[a];
+        fp->w64[a].assign((size_t)rows * K, 0.0);
+        for (int r = 0; r < rows; ++r) {
+            const int src = X.uniform ? 0 : r;
+            if (!X.uniform && (r < fp->nat_lo[a] || r >= fp->nat_hi[a])) continue;
+            for (int k = 0; k < X.K; ++k)
+                fp->w64[a][(size_t)r * K + (X.dmin + k + half)] = T.w64[(size_t)src * X.K + k];
+        }
+        fp->w32[a].resize(fp->w64[a].size());
+        for (size_t i = 0; i < fp->w64[a].size(); ++i) fp->w32[a][i] = (float)fp->w64[a][i];
+        A.uniform[a] = X.uniform;
+    }
+    // shells: the domain minus the box, slabs from the slowest axis down
+    fp->shell_total = shell_boxes(P, A.lo, A.hi, &fp->shell);
+    (void)shell_boxes(P, fp->nat_lo, fp->nat_hi, &fp->nat_shell);
     fp->kind = P.ndim;
 }
 
@@ -115,6 +126,7 @@ bool wide_ok(const FastPlan& fp) { return fp.VX > 1 && fp.args.n[0] % (2 * fp.VX
 
 int fast_variant_count(const FastPlan& fp)
 {
+    if (fp.kind == 1) return 1;
     if (fp.kind == 2) return wide_ok(fp) ? 32 : 16;
     if (fp.kind == 3) return fp.VX > 1 ? 16 : 8;
     return 1;
@@ -122,7 +134,7 @@ int fast_variant_count(const FastPlan& fp)
 
 bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
 {
-    if (fp.kind == 0 || v < 0 || v >= fast_variant_count(fp)) return false;
+    if (fp.kind < 2 || v < 0 || v >= fast_variant_count(fp)) return false;
     FastArgs A = fp.args;
     A.xcd_remap = (v & 1) ? 0 : 1;
     if (fp.kind == 2) {

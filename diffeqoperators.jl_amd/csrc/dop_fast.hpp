@@ -45,12 +45,17 @@ struct FastArgs {
 };
 
 struct FastPlan {
-    int kind = 0;          // 0: generic only, 2: 2-D tiled, 3: 3-D tiled
+    int kind = 0;          // 0: generic only, 1: 1-D window tables only (batched tiles),
+                           // 2: 2-D tiled, 3: 3-D tiled
     int VX = 1;            // widest points-per-lane the row alignment allows
     int kahan = 0;         // compensated f32 accumulation: batched tiles only
     FastArgs args{};
     ShellArgs shell{};
     int64_t shell_total = 0;
+    // the box before the x edges are aligned for the single-plan kernels' vector stores, and
+    // its shell (the batched tiles store single points and use these)
+    int32_t nat_lo[3] = {0, 0, 0}, nat_hi[3] = {1, 1, 1};
+    ShellArgs nat_shell{};
     std::vector<double> w64[3];   // host copies before upload
     std::vector<float> w32[3];
 };

@@ -62,11 +62,15 @@ static hipError_t launch_generic_any(const PlanDev& host, const PlanDev* dev, vo
         if (host.accum == kAccF64) return launch_generic<float, kAccF64>(host, dev, du, u, accumulate, s);
         if (host.accum == kAccF32Kahan)
             return launch_generic<float, kAccF32Kahan>(host, dev, du, u, accumulate, s);
+        if (host.accum == kAccF32Wide)
+            return launch_generic<float, kAccF32Wide>(host, dev, du, u, accumulate, s);
         return launch_generic<float, kAccF32>(host, dev, du, u, accumulate, s);
     case kBF16:
         if (host.accum == kAccF64) return launch_generic<bf16, kAccF64>(host, dev, du, u, accumulate, s);
         if (host.accum == kAccF32Kahan)
             return launch_generic<bf16, kAccF32Kahan>(host, dev, du, u, accumulate, s);
+        if (host.accum == kAccF32Wide)
+            return launch_generic<bf16, kAccF32Wide>(host, dev, du, u, accumulate, s);
         return launch_generic<bf16, kAccF32>(host, dev, du, u, accumulate, s);
     }
     return hipErrorInvalidValue;
@@ -77,7 +81,7 @@ hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumu
 {
     // the tiled kernels use 16-byte lane loads/stores: both arrays must be 16-byte aligned
     const bool aligned = ((uintptr_t)du % 16 == 0) && ((uintptr_t)u % 16 == 0);
-    if (p.fast.kind != 0 && !p.fast.kahan && !force_generic && aligned)
+    if (p.fast.kind >= 2 && !p.fast.kahan && !force_generic && aligned)
         return launch_tiled(p, du, u, accumulate, variant < 0 ? p.variant : variant, s);
     return launch_generic_any(p.host, p.dev, du, u, accumulate, s);
 }

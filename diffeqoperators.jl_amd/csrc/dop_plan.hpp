@@ -25,7 +25,7 @@ constexpr int kMaxBCTaps = 8;
 constexpr int kMaxDim = 3;
 
 enum Storage : int32_t { kF64 = 0, kF32 = 1, kBF16 = 2 };
-enum Accum : int32_t { kAccF64 = 1, kAccF32 = 2, kAccF32Kahan = 3 };
+enum Accum : int32_t { kAccF64 = 1, kAccF32 = 2, kAccF32Kahan = 3, kAccF32Wide = 4 };
 
 struct AxisDev {
     int32_t n;        // extent along the axis

@@ -15,7 +15,7 @@ ABI_VERSION = 1
 
 OK, EINVAL, EDIM, EHIP, ENOMEM, EUNSUPPORTED, EBUDGET = 0, -1, -2, -3, -4, -5, -6
 F64, F32, BF16 = 0, 1, 2
-ACC_AUTO, ACC_F64, ACC_F32, ACC_F32_KAHAN = 0, 1, 2, 3
+ACC_AUTO, ACC_F64, ACC_F32, ACC_F32_KAHAN, ACC_F32_WIDE = 0, 1, 2, 3, 4
 INPUT_GHOST, INPUT_PADDED = 0, 1
 SEM_LINES, SEM_BPV = 0, 1
 COEFF_VECTOR, COEFF_NUM_POS, COEFF_NUM_NEG = 0, 1, 2

@@ -50,10 +50,14 @@ extern "C" {
 #define DOP_BF16 2
 
 /* accumulation */
-#define DOP_ACC_AUTO 0        /* f64 for f64 storage, f32 (+Kahan when d >= 3) otherwise */
+#define DOP_ACC_AUTO 0        /* f64 for f64 storage; f32, or DOP_ACC_F32_WIDE when an operator
+                                 has d >= 3, for f32 / bf16 storage */
 #define DOP_ACC_F64 1
 #define DOP_ACC_F32 2
 #define DOP_ACC_F32_KAHAN 3   /* f32 with compensated (TwoProduct/TwoSum) accumulation */
+#define DOP_ACC_F32_WIDE 4    /* f32 weights and values; their products, exact in f64, are
+                                 summed in f64 (compensation for high derivative orders at a
+                                 third of the ALU work of DOP_ACC_F32_KAHAN) */
 
 /* input kinds */
 #define DOP_INPUT_GHOST 0     /* u unpadded; ghosts from the per-axis affine BCs (Q*u) */

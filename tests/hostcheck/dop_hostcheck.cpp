@@ -67,11 +67,13 @@ int hc_apply(const dop_desc* d, void* du, const void* u, int accumulate)
     case kF32:
         if (P.accum == kAccF64) run<float, kAccF64>(P, (float*)du, (const float*)u, accumulate);
         else if (P.accum == kAccF32Kahan) run<float, kAccF32Kahan>(P, (float*)du, (const float*)u, accumulate);
+        else if (P.accum == kAccF32Wide) run<float, kAccF32Wide>(P, (float*)du, (const float*)u, accumulate);
         else run<float, kAccF32>(P, (float*)du, (const float*)u, accumulate);
         break;
     case kBF16:
         if (P.accum == kAccF64) run<bf16, kAccF64>(P, (bf16*)du, (const bf16*)u, accumulate);
         else if (P.accum == kAccF32Kahan) run<bf16, kAccF32Kahan>(P, (bf16*)du, (const bf16*)u, accumulate);
+        else if (P.accum == kAccF32Wide) run<bf16, kAccF32Wide>(P, (bf16*)du, (const bf16*)u, accumulate);
         else run<bf16, kAccF32>(P, (bf16*)du, (const bf16*)u, accumulate);
         break;
     }

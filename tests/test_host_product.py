@@ -306,6 +306,13 @@ def test_hc_kahan_high_order(hc, rng):
     p = hc_apply(hc, [L], u, storage=_abi.F32, sem=_abi.SEM_BPV, bcs={0: bc}, accum=_abi.ACC_F32)
     assert rel(k, ref) < 1e-5
     assert rel(k, ref) <= rel(p, ref)
+    # exact f32 products summed in f64: at least as close, and what the automatic mode
+    # picks for d >= 3
+    w = hc_apply(hc, [L], u, storage=_abi.F32, sem=_abi.SEM_BPV, bcs={0: bc}, accum=_abi.ACC_F32_WIDE)
+    a = hc_apply(hc, [L], u, storage=_abi.F32, sem=_abi.SEM_BPV, bcs={0: bc}, accum=_abi.ACC_AUTO)
+    assert rel(w, ref) < 1e-5
+    assert rel(w, ref) <= rel(p, ref)
+    assert np.array_equal(w, a)
 
 
 def test_hc_plan_rejects_bad_shapes(hc, rng):
