@@ -11,6 +11,18 @@
  * axis 0 is the fastest-varying one.  All arithmetic is IEEE binary64 with no FMA
  * contraction (built with -ffp-contract=off), so table generation reproduces the
  * reference's operation order.
+ *
+ * What pins it (tests/test_oracle_golden.py): the analytic matrices and values of the
+ * reference's own tests (tests/golden/reference_fixtures.json), closed forms, invariants
+ * (weights sum to 0, polynomial exactness, reflected boundaries, max-offside upwind ==
+ * centered) and a planted mistake that must fail.  Parity unpinned (restated from the
+ * source only; no test, fixture or closed form of the reference fixes them):
+ *   - orc_mul_line_bpv for centered operators with non-constant coefficient vectors
+ *     (the coefficient index of convolutions.jl:384);
+ *   - orc_composite_mul_padded's transverse views 1:n on non-separable fields
+ *     (derivative_operator_functions.jl:429-460, 780-800);
+ *   - orc_bc.periodic == 2, the N-D PeriodicBC padding with the first / last planes
+ *     (multi_dim_bc_operators.jl:221-252).
  */
 #ifndef DOP_ORACLE_H
 #define DOP_ORACLE_H
