@@ -219,6 +219,26 @@ __device__ inline double dpp_shl1(double v)
     return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// Role of block i of a launch with `ntile` tile blocks and `nshell` shell blocks: the two are
+// interleaved in groups of 8 (one group of each per round over the 8 XCDs), so the
+// latency-bound shell runs beside the bandwidth-bound tiles and tile block k still lands on
+// XCD k mod 8.  Returns true for a tile (its index in *idx), false for a shell chunk.
+__device__ inline bool block_role(int i, int ntile, int nshell, int* idx)
+{
+    const int m = min(ntile, nshell) / 8 * 8;
+    if (i < 2 * m) {
+        *idx = (i >> 4) * 8 + (i & 7);
+        return (i & 15) < 8;
+    }
+    const int rest = i - 2 * m;
+    if (rest < ntile - m) {
+        *idx = m + rest;
+        return true;
+    }
+    *idx = m + (rest - (ntile - m));
+    return false;
+}
+
 // logical tile of this block: each XCD (blocks b, b+8, b+16, ...) gets a contiguous band
 __device__ inline int tile_of(int b, int nb, int remap)
 {
@@ -245,10 +265,19 @@ __device__ inline void shell_points(const PlanDev& P, const ShellArgs& S, T* __r
         int64_t l = p - S.start[b];
         const int64_t e0 = S.ext[b][0], e1 = S.ext[b][1];
         int64_t idx[3];
-        idx[0] = S.org[b][0] + l % e0;
-        l /= e0;
-        idx[1] = S.org[b][1] + l % e1;
-        idx[2] = S.org[b][2] + l / e1;
+        if (S.start[b + 1] - S.start[b] < ((int64_t)1 << 31)) {   // 32-bit divisions
+            const uint32_t l32 = (uint32_t)l, f0 = (uint32_t)e0, f1 = (uint32_t)e1;
+            const uint32_t q0 = l32 / f0;
+            idx[0] = S.org[b][0] + (l32 - q0 * f0);
+            const uint32_t q1 = q0 / f1;
+            idx[1] = S.org[b][1] + (q0 - q1 * f1);
+            idx[2] = S.org[b][2] + q1;
+        } else {
+            idx[0] = S.org[b][0] + l % e0;
+            l /= e0;
+            idx[1] = S.org[b][1] + l % e1;
+            idx[2] = S.org[b][2] + l / e1;
+        }
         const int64_t lin = idx[0] + idx[1] * P.sdu[1] + idx[2] * P.sdu[2];
         double v = eval_point<T, ACC>(P, src, idx);
         if (accumulate) v += (double)Store<T>::ld(du, lin);
@@ -310,14 +339,15 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     constexpr int NWIN = VX + 2 * H;
     constexpr int PADL = (H + VX - 1) / VX * VX;   // x halo, whole lanes
     constexpr int LEDGE = PADL / VX;               // load-only lanes at each end
-    if ((int)blockIdx.x < nshell) {
+    int bi;
+    if (!block_role((int)blockIdx.x, A.nblocks, nshell, &bi)) {
         const int64_t per = 256 * kShellPtsPerThread;
-        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
+        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)bi * per, per);
         return;
     }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int b = tile_of((int)blockIdx.x - nshell, A.nblocks, A.xcd_remap);
+    const int b = tile_of(bi, A.nblocks, A.xcd_remap);
     const int xg = b % A.gx;
     const int ch = b / A.gx;
     const int xseg = xg * 4 + wave;
@@ -432,15 +462,16 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     __shared__ __attribute__((aligned(16))) C lds[2][W][ROWW];
     __shared__ C wzs[UNI ? 1 : kMaxChunk3D][K];
     __shared__ __attribute__((aligned(16))) C wxs[UNI ? 1 : K][UNI ? 1 : ROWW];   // per column
-    if ((int)blockIdx.x < nshell) {   // whole block: no barrier is reached
+    int bi;
+    if (!block_role((int)blockIdx.x, A.nblocks, nshell, &bi)) {   // whole block: no barrier
         const int64_t per = 64 * W * kShellPtsPerThread;
-        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * per, per);
+        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)bi * per, per);
         return;
     }
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int b = tile_of((int)blockIdx.x - nshell, A.nblocks, A.xcd_remap);
+    const int b = tile_of(bi, A.nblocks, A.xcd_remap);
     const int xseg = b % A.nxseg;
     const int t = b / A.nxseg;
     const int yblk = t % A.nyblk;
@@ -563,6 +594,17 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     }
 }
 
+// The shell of a 3-D plan: one point per thread in small blocks.  This per-point work is
+// latency-bound; in its own launch it runs at full occupancy instead of inheriting the tile
+// kernel's 1024-thread blocks, registers and LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_shell(const PlanDev* __restrict__ Pp, const ShellArgs SH,
+                                             T* __restrict__ du, const T* __restrict__ u,
+                                             int accumulate)
+{
+    shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * 256, 256);
+}
+
 // ---------------------------------------------------------------------------------------
 template <typename T, int VX, int K>
 hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
@@ -573,9 +615,15 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
     // per-row indexing needs a row of weights for every point of every axis
     for (int a = 0; a < fp.kind; ++a)
         if (fp.w64[a].size() != (size_t)(uni ? 1 : A.n[a]) * K) return hipErrorInvalidValue;
-    const int threads = fp.kind == 2 ? 256 : 64 * A.W;
-    const int64_t per = (int64_t)threads * kShellPtsPerThread;
-    const int nshell = (int)((fp.shell_total + per - 1) / per);
+    const int64_t per = (int64_t)256 * kShellPtsPerThread;
+    // 2-D: the (thin) shell rides in the tile launch; 3-D: a launch of its own first
+    const int nshell = fp.kind == 2 ? (int)((fp.shell_total + per - 1) / per) : 0;
+    if (fp.kind == 3 && fp.shell_total > 0) {
+        hipLaunchKernelGGL((k_shell<T>), dim3((unsigned)((fp.shell_total + 255) / 256)), dim3(256), 0,
+                           s, p.dev, fp.shell, (T*)du, (const T*)u, accumulate);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     const dim3 grid((unsigned)(A.nblocks + nshell));
     if (fp.kind == 2) {
 #define DOP_L2(UNI, PF)                                                                      \
