@@ -128,7 +128,7 @@ int fast_variant_count(const FastPlan& fp)
 {
     if (fp.kind == 1) return 1;
     if (fp.kind == 2) return wide_ok(fp) ? 32 : 16;
-    if (fp.kind == 3) return fp.VX > 1 ? 16 : 8;
+    if (fp.kind == 3) return fp.VX > 1 ? 32 : 16;
     return 1;
 }
 
@@ -143,13 +143,15 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         A.PF = ((v >> 3) & 1) ? 4 : 2;
         A.VX = ((v >> 4) & 1) ? 2 * fp.VX : fp.VX;
         A.W = 4;
+        A.VY = 1;
     } else {
         static const int chunks[2] = {64, 32};
         A.chunk = chunks[(v >> 1) & 1];
         A.PF = 1;
         A.W = ((v >> 2) & 1) ? 16 : 8;
-        if (2 * (A.K / 2) >= 8) A.W = 16;   // an 8-wave block must compute at least two rows
-        A.VX = ((v >> 3) & 1) ? fp.VX / 2 : fp.VX;
+        A.VY = ((v >> 3) & 1) ? 2 : 1;
+        if (A.VY == 1 && 2 * (A.K / 2) >= 8) A.W = 16;   // a block must compute some rows
+        A.VX = ((v >> 4) & 1) ? fp.VX / 2 : fp.VX;
     }
     {
         // segments of 64*VX points overlapping by 2*PADL (the x halo, whole lanes)
@@ -165,7 +167,7 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         A.nblocks = (int)(A.gx * nch);
     } else {
         // W waves stage W rows and compute the middle W - 2H
-        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], A.W - 2 * (A.K / 2));
+        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], A.W * A.VY - 2 * (A.K / 2));
         const int64_t nzc = ceil_div(A.hi[2] - A.lo[2], A.chunk);
         A.gx = 0;
         A.nblocks = (int)(A.nxseg * A.nyblk * nzc);

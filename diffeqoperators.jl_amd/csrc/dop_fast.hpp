@@ -29,7 +29,8 @@ struct FastArgs {
     int32_t K;             // window taps on every axis
     int32_t VX;            // points per lane along x (16-, 8- or 4-byte lane loads)
     int32_t PF;            // rows / planes prefetched ahead in the register ring
-    int32_t W;             // 3-D: waves (= y rows) per block
+    int32_t W;             // 3-D: waves per block
+    int32_t VY;            // 3-D: rows staged per wave (a block stages W*VY rows)
     int32_t lo[3], hi[3];  // interior box [lo, hi) per axis
     int32_t xbase;         // x of the first segment's first point (VX-aligned; 3-D: may be
                            // negative, the segment's first PADL points are x halo)

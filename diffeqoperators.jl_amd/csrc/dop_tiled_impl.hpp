@@ -444,7 +444,7 @@ constexpr int kMaxChunk3D = 64;
 template <typename T, int K>
 constexpr int kPF3D = sizeof(T) == 8 ? 1 : K <= 3 ? 4 : K <= 7 ? 2 : 1;
 
-template <typename T, int VX, int K, bool UNI, int PF, int W>
+template <typename T, int VX, int K, bool UNI, int PF, int W, int VY>
 __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const PlanDev* __restrict__ Pp,
                                                      const ShellArgs SH, int nshell,
                                                      T* __restrict__ du, const T* __restrict__ u,
@@ -453,13 +453,14 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     using C = typename Store<T>::C;
     constexpr int H = K / 2;
     constexpr int R = K + PF;
+    constexpr int SR = W * VY;                            // staged rows per block
     constexpr int ROWW = 64 * VX;
     constexpr int PADL = (H + VX - 1) / VX * VX;          // x halo, whole lanes
     constexpr int LEDGE = PADL / VX;                      // load-only lanes at each end
     constexpr int NRD = (PADL + H + VX + VX - 1) / VX;    // VX-wide reads covering a window
-    static_assert(2 * H < W, "a block computes at least one row");
+    static_assert(2 * H < SR, "a block computes at least one row");
     static_assert((64 - 2 * LEDGE - 1) * VX + NRD * VX <= ROWW, "x window inside the row");
-    __shared__ __attribute__((aligned(16))) C lds[2][W][ROWW];
+    __shared__ __attribute__((aligned(16))) C lds[2][SR][ROWW];
     __shared__ C wzs[UNI ? 1 : kMaxChunk3D][K];
     __shared__ __attribute__((aligned(16))) C wxs[UNI ? 1 : K][UNI ? 1 : ROWW];   // per column
     int bi;
@@ -476,9 +477,9 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     const int t = b / A.nxseg;
     const int yblk = t % A.nyblk;
     const int zch = t / A.nyblk;
-    const int y = A.lo[1] + yblk * (W - 2 * H) - H + wave;   // the row this wave stages
-    const bool computes = wave >= H && wave < W - H;         // wave-uniform
-    const bool yok = computes && y < A.hi[1];
+    // staged row r of the block is y = yb0 + r; wave w stages rows w + i*W (i < VY) and
+    // computes those in [H, SR - H)
+    const int yb0 = A.lo[1] + yblk * (SR - 2 * H) - H;
     const int z0 = A.lo[2] + zch * A.chunk;
     const int z1 = min(z0 + A.chunk, A.hi[2]);
     const int64_t n0 = A.n[0];
@@ -492,20 +493,31 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     // box, i.e. never outside the array, so those values are never used.
     const int64_t xl = min(max(xs, (int64_t)0), n0 - VX);  // clamped load column (rows are
                                                            // whole VX groups)
-    const bool st_ok = yok && lane >= LEDGE && lane < 64 - LEDGE && xs >= A.lo[0] && xs < A.hi[0];
-    const int yc = min(max(y, 0), n1 - 1);
+    const bool xst = lane >= LEDGE && lane < 64 - LEDGE && xs >= A.lo[0] && xs < A.hi[0];
     const int lr = min(max(lane, LEDGE), 63 - LEDGE);      // window reads of load-only lanes
     const int wcol = (lr - LEDGE) * VX;                     // first column of the window reads
+
+    int row[VY], y[VY], yc[VY];
+    bool computes[VY], st_ok[VY];
+#pragma unroll
+    for (int i = 0; i < VY; ++i) {
+        row[i] = wave + i * W;
+        y[i] = yb0 + row[i];
+        yc[i] = min(max(y[i], 0), n1 - 1);
+        computes[i] = row[i] >= H && row[i] < SR - H;      // wave-uniform
+        st_ok[i] = computes[i] && y[i] < A.hi[1] && xst;
+    }
 
     const C* wxt = wtab<C>(A, 0);
     const C* wyt = wtab<C>(A, 1);
     const C* wzt = wtab<C>(A, 2);
     C wxu[K], wzu[K];
-    C wy[K];
-    {
-        const int64_t yr = UNI ? 0 : yc;
+    C wy[VY][K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) wy[k] = wyt[yr * K + k];
+    for (int i = 0; i < VY; ++i) {
+        const int64_t yr = UNI ? 0 : yc[i];
+#pragma unroll
+        for (int k = 0; k < K; ++k) wy[i][k] = wyt[yr * K + k];
     }
     if constexpr (UNI) {
 #pragma unroll
@@ -525,15 +537,20 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     }
 
     // one VX-wide load per call, on every path, straight into its ring slot
-    const T* urow = u + (int64_t)yc * s1 + xl;
-    auto load_plane = [&](int zz, C* dst) {
+    const T* urow[VY];
+#pragma unroll
+    for (int i = 0; i < VY; ++i) urow[i] = u + (int64_t)yc[i] * s1 + xl;
+    auto load_plane = [&](int i, int zz, C* dst) {
         zz = min(max(zz, 0), n2 - 1);
-        VecIO<T, VX>::ld(urow + (int64_t)zz * s2, dst);
+        VecIO<T, VX>::ld(urow[i] + (int64_t)zz * s2, dst);
     };
 
-    C q[R][VX];
+    C q[VY][R][VX];
 #pragma unroll
-    for (int s = 0; s < R - 1; ++s) load_plane(z0 - H + s, q[s]);
+    for (int s = 0; s < R - 1; ++s) {
+#pragma unroll
+        for (int i = 0; i < VY; ++i) load_plane(i, z0 - H + s, q[i][s]);
+    }
 
     int buf = 0;
     for (int zb = z0; zb < z1; zb += R) {
@@ -541,22 +558,28 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
             constexpr int j = decltype(jc)::value;
             const int z = zb + j;
             if (z >= z1) return false;   // block-uniform
-            load_plane(z + H + PF, q[(j + K - 1 + PF) % R]);
 #pragma unroll
-            for (int v = 0; v < VX; ++v) lds[buf][wave][lane * VX + v] = q[(j + H) % R][v];
+            for (int i = 0; i < VY; ++i) load_plane(i, z + H + PF, q[i][(j + K - 1 + PF) % R]);
+#pragma unroll
+            for (int i = 0; i < VY; ++i) {
+#pragma unroll
+                for (int v = 0; v < VX; ++v) lds[buf][row[i]][lane * VX + v] = q[i][(j + H) % R][v];
+            }
             __syncthreads();
-            if (computes) {
+            C wz[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if constexpr (UNI) wz[k] = wzu[k];
+                else wz[k] = wzs[z - z0][k];
+            }
+#pragma unroll
+            for (int i = 0; i < VY; ++i) {
+                if (!computes[i]) continue;   // wave-uniform
                 C xw[NRD * VX];
 #pragma unroll
                 for (int r = 0; r < NRD; ++r) {
 #pragma unroll
-                    for (int v = 0; v < VX; ++v) xw[r * VX + v] = lds[buf][wave][wcol + r * VX + v];
-                }
-                C wz[K];
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if constexpr (UNI) wz[k] = wzu[k];
-                    else wz[k] = wzs[z - z0][k];
+                    for (int v = 0; v < VX; ++v) xw[r * VX + v] = lds[buf][row[i]][wcol + r * VX + v];
                 }
                 C out[VX];
 #pragma unroll
@@ -571,13 +594,13 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
                     }
 #pragma unroll
                     for (int k = 0; k < K; ++k)
-                        s = fmac<C>(wy[k], lds[buf][wave - H + k][lane * VX + v], s);
+                        s = fmac<C>(wy[i][k], lds[buf][row[i] - H + k][lane * VX + v], s);
 #pragma unroll
-                    for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[(j + k) % R][v], s);
+                    for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[i][(j + k) % R][v], s);
                     out[v] = s;
                 }
-                if (st_ok) {
-                    T* po = du + (int64_t)z * s2 + (int64_t)y * s1 + xs;
+                if (st_ok[i]) {
+                    T* po = du + (int64_t)z * s2 + (int64_t)y[i] * s1 + xs;
                     if (accumulate) {
                         C old[VX];
                         VecIO<T, VX>::ld(po, old);
@@ -637,23 +660,28 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
 #undef DOP_L2
     } else {
         if constexpr (K <= 9 && VX <= 16 / (int)sizeof(T) && VX <= 4) {
-#define DOP_L3(UNI, W)                                                                       \
-    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, kPF3D<T, K>, W>), grid, dim3(64 * W), 0, s, A,    \
-                       p.dev,                                                                  \
-                       fp.shell, nshell, (T*)du, (const T*)u, accumulate)
-            constexpr bool w8 = 2 * (K / 2) < 8;   // 8-wave blocks compute 8 - 2H rows
-            if (A.W != 16 && !w8) return hipErrorInvalidValue;
-            if (uni) {
-                if constexpr (w8) {
-                    if (A.W == 8) DOP_L3(true, 8); else DOP_L3(true, 16);
+#define DOP_L3(UNI, W, VY)                                                                   \
+    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, kPF3D<T, K>, W, VY>), grid, dim3(64 * W), 0, s, \
+                       A, p.dev, fp.shell, nshell, (T*)du, (const T*)u, accumulate)
+            constexpr bool w8 = 2 * (K / 2) < 8;   // 8-wave blocks of one row compute 8 - 2H rows
+            if (A.W != 16 && !w8 && A.VY == 1) return hipErrorInvalidValue;
+            if (A.VY == 2) {
+                if (uni) {
+                    if (A.W == 8) DOP_L3(true, 8, 2); else DOP_L3(true, 16, 2);
                 } else {
-                    DOP_L3(true, 16);
+                    if (A.W == 8) DOP_L3(false, 8, 2); else DOP_L3(false, 16, 2);
+                }
+            } else if (uni) {
+                if constexpr (w8) {
+                    if (A.W == 8) DOP_L3(true, 8, 1); else DOP_L3(true, 16, 1);
+                } else {
+                    DOP_L3(true, 16, 1);
                 }
             } else {
                 if constexpr (w8) {
-                    if (A.W == 8) DOP_L3(false, 8); else DOP_L3(false, 16);
+                    if (A.W == 8) DOP_L3(false, 8, 1); else DOP_L3(false, 16, 1);
                 } else {
-                    DOP_L3(false, 16);
+                    DOP_L3(false, 16, 1);
                 }
             }
 #undef DOP_L3

@@ -117,7 +117,7 @@ def n_variants(shape, dtype):
     vx = vx if n0 % vx == 0 else 1
     if len(shape) == 2:
         return 32 if vx > 1 and n0 % (2 * vx) == 0 else 16
-    return 16 if vx > 1 else 8
+    return 32 if vx > 1 else 16
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
