@@ -227,5 +227,60 @@ def _periodic_dim(dim, T, s):
 PeriodicBC = _DimBC(_periodic, _periodic_dim)
 
 
-def GeneralBC(*args, **kwargs):
-    raise NotImplementedError("GeneralBC is not on the accelerated path of this library")
+def _ipow(x, i):
+    r = x
+    for _ in range(1, i):
+        r = r * x
+    return r
+
+
+def _general_side(alpha, dx, order, right):
+    """One end of GeneralBC (bc_operators.jl:85-170): alpha[0] + alpha[1] u + sum_i alpha[i+1]
+    u^(i) = 0, the i-th derivative taken on the points 1..order+i (at point 1 on the left, at
+    point order+i on the right), rows zero-padded at the end as in the reference."""
+    alpha = [float(v) for v in alpha]
+    n = len(alpha)
+    ncol = order + n - 2
+    vec = np.ndim(dx) != 0
+    dxv = np.asarray(dx, dtype=np.float64) if vec else float(dx)
+    S = []
+    for i in range(1, n - 1):
+        m = order + i
+        w = calculate_weights(i, float(m) if right else 1.0, [float(k) for k in range(1, m + 1)])
+        row = []
+        for j in range(ncol):
+            v = float(w[j]) if j < m else 0.0
+            if vec:
+                h = _ipow(float(dxv[len(dxv) - 1 - j]) if right else float(dxv[j]), i)
+            else:
+                h = _ipow(dxv, i)
+            row.append(_div(v, h))
+        S.append(row)
+    denom = alpha[1]
+    for i in range(n - 2):
+        denom = denom + alpha[i + 2] * S[i][ncol - 1 if right else 0]
+    nt = ncol - 1
+    t = []
+    for j in range(nt):
+        col = (ncol - 2 - j) if right else (j + 1)
+        acc = 0.0
+        for i in range(n - 2):
+            acc = acc + alpha[i + 2] * S[i][col]
+        t.append(_div(-acc, denom))
+    a = list(reversed(t)) if right else t
+    return a, _div(-alpha[0], denom)
+
+
+def GeneralBC(alpha_l, alpha_r, dx, order=1):
+    """GeneralBC(αl, αr, dx, order) (bc_operators.jl:85-170): an affine BC whose ghost solves
+    αl[1] + αl[2] u + αl[3] u' + αl[4] u'' + ... = 0 at the left end (likewise at the right)."""
+    if len(alpha_l) < 2 or len(alpha_r) < 2:
+        raise ValueError("GeneralBC needs at least two coefficients per side")
+    if np.ndim(dx) != 0:
+        need = max(order + len(alpha_l) - 2, order + len(alpha_r) - 2)
+        if len(dx) < need:
+            raise ValueError("GeneralBC: dx vector too short")
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a_l, b_l = _general_side(alpha_l, dx, order, False)
+        a_r, b_r = _general_side(alpha_r, dx, order, True)
+    return AffineBC(a_l, b_l, a_r, b_r)

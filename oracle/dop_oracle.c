@@ -293,6 +293,62 @@ int orc_robin_nonuniform(double al, double bl, double cl, double ar, double br, 
     return ORC_OK;
 }
 
+/* bc_operators.jl:85-170 -- GeneralBC.  Row i (1-based, i <= n-2) of S holds the weights of
+ * the i-th derivative on the points 1..order+i (at point 1 on the left, at point order+i on
+ * the right), zero-padded to order+n-2 columns and divided by dx^i (per column for a vector
+ * step); the ghost solves the BC with the ghost's column (s0) split off. */
+static double ipow(double x, int i)
+{
+    double r = x;
+    for (int k = 1; k < i; ++k) r *= x;
+    return r;
+}
+
+static int general_side(const double* alpha, int n, const double* dx, int ndx, int order,
+                        int right, double* a, double* b)
+{
+    enum { MAXC = 32 };
+    const int ncol = order + n - 2;   /* columns of S */
+    if (n < 2 || order < 1 || ncol > MAXC || n - 2 > MAXC) return ORC_EINVAL;
+    if (ndx > 0 && ndx < ncol) return ORC_EINVAL;
+    double S[MAXC][MAXC];
+    double xs[MAXC], w[MAXC];
+    for (int i = 1; i <= n - 2; ++i) {
+        const int m = order + i;
+        for (int k = 0; k < m; ++k) xs[k] = (double)(k + 1);
+        if (orc_calculate_weights(i, right ? (double)m : 1.0, xs, m, w) != ORC_OK) return ORC_EINVAL;
+        for (int j = 0; j < ncol; ++j) {
+            const double v = j < m ? w[j] : 0.0;
+            double h;
+            if (ndx == 0) h = ipow(dx[0], i);
+            else h = ipow(right ? dx[ndx - 1 - j] : dx[j], i);   /* dx_r = reverse(dx[end-ncol+1:end]) */
+            S[i - 1][j] = v / h;
+        }
+    }
+    /* s0 = S[:, 1] (left) or S[:, end] (right); the rest Sl = S[:, 2:end] or Sr = S[:, end-1:-1:1] */
+    double denom = alpha[1];
+    for (int i = 0; i < n - 2; ++i) denom += alpha[i + 2] * S[i][right ? ncol - 1 : 0];
+    const int nt = ncol - 1;
+    double t[MAXC];
+    for (int j = 0; j < nt; ++j) {
+        const int col = right ? (ncol - 2 - j) : (j + 1);
+        double acc = 0.0;
+        for (int i = 0; i < n - 2; ++i) acc += alpha[i + 2] * S[i][col];
+        t[j] = -acc / denom;
+    }
+    for (int j = 0; j < nt; ++j) a[j] = right ? t[nt - 1 - j] : t[j];   /* a_r = reverse(...) */
+    *b = -alpha[0] / denom;
+    return ORC_OK;
+}
+
+int orc_general(const double* alpha_l, int nl, const double* alpha_r, int nr, const double* dx,
+                int ndx, int order, double* a_l, double* b_l, double* a_r, double* b_r)
+{
+    int r = general_side(alpha_l, nl, dx, ndx, order, 0, a_l, b_l);
+    if (r == ORC_OK) r = general_side(alpha_r, nr, dx, ndx, order, 1, a_r, b_r);
+    return r;
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Line kernels.  P(j) is the 1-based padded vector, OUT(i) the 1-based output. */
 #define P(j) x[(j) - 1]

@@ -103,6 +103,12 @@ int orc_robin_uniform(double al, double bl, double cl, double ar, double br, dou
                       double dx, int order, double* a_l, double* b_l, double* a_r,
                       double* b_r);
 /* bc_operators.jl:40-60 ; dx has ndx >= order+1 entries */
+/* bc_operators.jl:85-170 -- GeneralBC: alpha_l[0] + alpha_l[1] u + sum_i alpha_l[i+1] u^(i) = 0
+ * at each end (nl, nr coefficients); ndx == 0: uniform step dx[0], else dx[ndx].
+ * a_l gets order + nl - 3 taps, a_r order + nr - 3. */
+int orc_general(const double* alpha_l, int nl, const double* alpha_r, int nr, const double* dx,
+                int ndx, int order, double* a_l, double* b_l, double* a_r, double* b_r);
+
 int orc_robin_nonuniform(double al, double bl, double cl, double ar, double br, double cr,
                          const double* dx, int ndx, int order, double* a_l, double* b_l,
                          double* a_r, double* b_r);

@@ -59,6 +59,7 @@ def lib():
         L.orc_upwind_nonuniform.argtypes = [i, i, _dp, i, i, _dp, _dp, _dp]
         L.orc_robin_uniform.argtypes = [d, d, d, d, d, d, d, i, _dp, _dp, _dp, _dp]
         L.orc_robin_nonuniform.argtypes = [d, d, d, d, d, d, _dp, i, i, _dp, _dp, _dp, _dp]
+        L.orc_general.argtypes = [_dp, i, _dp, i, _dp, i, i, _dp, _dp, _dp, _dp]
         L.orc_mul_line.argtypes = [C.POINTER(OrcOp), _dp, _dp, i]
         L.orc_mul_line_bpv.argtypes = [C.POINTER(OrcOp), d, d, _dp, _dp, i]
         L.orc_mul_nd.argtypes = [C.POINTER(OrcOp), i, llp, _dp, llp, _dp, i]
@@ -225,6 +226,24 @@ def dirichlet(vl, vr) -> BC:
 
 def dirichlet0() -> BC:
     return dirichlet(0.0, 0.0)
+
+
+def general(alpha_l, alpha_r, dx, order=1) -> BC:
+    """GeneralBC(αl, αr, dx, order): bc_operators.jl:85-170."""
+    al = np.ascontiguousarray(alpha_l, dtype=np.float64)
+    ar = np.ascontiguousarray(alpha_r, dtype=np.float64)
+    a_l = np.zeros(max(order + len(al) - 3, 0))
+    a_r = np.zeros(max(order + len(ar) - 3, 0))
+    b_l = C.c_double()
+    b_r = C.c_double()
+    if np.ndim(dx) == 0:
+        dxv, ndx = np.array([float(dx)]), 0
+    else:
+        dxv = np.ascontiguousarray(dx, dtype=np.float64)
+        ndx = len(dxv)
+    _check(lib().orc_general(_ptr(al), len(al), _ptr(ar), len(ar), _ptr(dxv), ndx, order,
+                             _ptr(a_l), C.byref(b_l), _ptr(a_r), C.byref(b_r)), "general")
+    return BC(a_l, b_l.value, a_r, b_r.value)
 
 
 def periodic() -> BC:

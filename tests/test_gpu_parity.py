@@ -282,3 +282,40 @@ def test_periodic_nd(dtype, form, shape):
     got = host64(A * u)
     ref = po.ghost_apply([oop(L) for L in ops], obc_any(Q, nd, js), host64(u))
     assert rel(got, ref) < TOL[dtype]
+
+
+# ---------------------------------------------------------------------------------------
+# GeneralBC (bc_operators.jl:85-170), as in KdV.jl:28: third derivative with a BC on u, u', u'''
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_general_bc_kdv(dtype):
+    n = 400
+    dx = 20.0 / (n + 1)
+    rng = np.random.default_rng(31)
+    bc = D.GeneralBC([0.0, 1.0, -0.6, 0.0, -1.0], [0.0, 1.0, -0.4, 0.0, -1.0], dx, 3)
+    L3 = D.CenteredDifference(3, 2, dx, n)
+    L1 = D.UpwindDifference(1, 1, dx, n, -rng.standard_normal(n))
+    A = (L3 + L1) * bc
+    u = rng.uniform(-1, 1, n)
+    print(f"case: GeneralBC KdV 1-D {dtype}", flush=True)
+    ud = to_dev(u, dtype)
+    got = host64(A * ud)
+    ref = po.ghost_apply([oop(L3), oop(L1)], [po.general([0.0, 1.0, -0.6, 0.0, -1.0],
+                                                        [0.0, 1.0, -0.4, 0.0, -1.0], dx, 3)],
+                         host64(ud), bpv_1d=True)
+    assert rel(got, ref) < TOL[dtype]
+
+
+def test_general_bc_2d():
+    n = 300
+    h = 1.0 / (n + 1)
+    rng = np.random.default_rng(32)
+    atom = D.GeneralBC([0.3, 1.0, 0.5, 0.1], [-0.2, 1.0, -0.5, 0.2], h, 2)
+    Q = D.compose(*D.MultiDimBC(atom, (n, n)))
+    ops = [D.CenteredDifference[1](2, 4, h, n), D.CenteredDifference[2](2, 4, h, n)]
+    A = (ops[0] + ops[1]) * Q
+    u = rng.uniform(-1, 1, (n, n))
+    print("case: GeneralBC 2-D f64", flush=True)
+    got = host64(A * to_dev(u))
+    ref = po.ghost_apply([oop(L) for L in ops], [obc(atom), obc(atom)], u)
+    assert rel(got, ref) < 1e-12

@@ -91,8 +91,8 @@ def test_constructor_errors():
         D.compose(Qs[0], Qs[0])
     with pytest.raises(ValueError):
         D.compose(Qs[0])
-    with pytest.raises(NotImplementedError):
-        D.GeneralBC([1.0, 2.0, 3.0], 0.1, 2)
+    with pytest.raises(ValueError):
+        D.GeneralBC([1.0], [1.0, 2.0], 0.1, 2)
 
 
 def test_scaling_keeps_coeff_func():
@@ -371,4 +371,40 @@ def test_hc_nd_periodic(hc, rng, form, shape):
     terms = [D.CenteredDifference[N](2, 4, 0.1, js[N - 1]) for N in range(1, nd + 1)]
     got = hc_apply(hc, terms, u, sem=sem, bcs=bcs)
     ref = po.ghost_apply([_oracle_op(L, L.N - 1) for L in terms], [_obc(bcs[k]) for k in range(nd)], u)
+    assert rel(got, ref) < 1e-13
+
+
+# ---------------------------------------------------------------------------------------
+# GeneralBC: tables equal the oracle's bit for bit; applied through the device arithmetic
+
+GENERAL_CASES = [([-10.0, 1.0, 6.0], [-10.0, 1.0, 6.0], 1.0, 3),
+                 ([0.0, 1.0, -0.3, 0.0, -1.0], [0.0, 1.0, -0.2, 0.0, -1.0], 0.05, 3),   # KdV.jl:28
+                 ([0.5, 1.0, 2.0, 3.0], [0.2, 1.0, -1.0, 0.5], "vector", 2)]
+
+
+@pytest.mark.parametrize("al,ar,dx,order", GENERAL_CASES)
+def test_general_bc_tables_match_oracle(al, ar, dx, order):
+    if dx == "vector":
+        dx = np.linspace(0.1, 0.2, 20)
+    g = D.GeneralBC(al, ar, dx, order)
+    q = po.general(al, ar, dx, order)
+    assert np.array_equal(g.a_l, q.a_l) and np.array_equal(g.a_r, q.a_r)
+    assert g.b_l == q.b_l and g.b_r == q.b_r
+
+
+@pytest.mark.parametrize("al,ar,dx,order", GENERAL_CASES)
+def test_hc_general_bc(hc, rng, al, ar, dx, order):
+    n = 30
+    dxv = np.linspace(0.1, 0.2, n + 1) if dx == "vector" else dx
+    bc = D.GeneralBC(al, ar, dxv, order)
+    u = rng.standard_normal(n)
+    L = D.CenteredDifference(3, 2, dxv, n) if dx != "vector" else D.CenteredDifference(2, 2, dxv, n)
+    got = hc_apply(hc, [L], u, sem=_abi.SEM_BPV, bcs={0: bc})
+    ref = po.ghost_apply([_oracle_op(L, 0)], [_obc(bc)], u, bpv_1d=True)
+    assert rel(got, ref) < 1e-13
+    # 2-D: the same atom extended over both axes
+    U = rng.standard_normal((n, n))
+    ops = [D.CenteredDifference[1](2, 2, dxv, n), D.CenteredDifference[2](2, 2, dxv, n)]
+    got = hc_apply(hc, ops, U, bcs={0: bc, 1: bc})
+    ref = po.ghost_apply([_oracle_op(L2, L2.N - 1) for L2 in ops], [_obc(bc), _obc(bc)], U)
     assert rel(got, ref) < 1e-13

@@ -307,3 +307,47 @@ def test_periodic_nd_padding_is_edge_planes(rng):
         line = u[j]
         assert rel(nd[j], L.mul_padded(np.concatenate([[line[0]], line, [line[-1]]]))) < 1e-15
         assert rel(wrap[j], L.mul_padded(np.concatenate([[line[-1]], line, [line[0]]]))) < 1e-15
+
+
+# ---------------------------------------------------------------------------------------
+# GeneralBC
+
+def test_general_bc_equals_robin(fixtures):
+    # robin.jl:92-110: GeneralBC([-10, 1, 6], [-10, 1, 6], dx, 3) pads u = 1:10 with the same
+    # ghosts as RobinBC((1, 6, 10), (1, 6, 10), dx, 3): -4/10 and 125/12, uniform and vector dx
+    u = np.arange(1.0, 11.0)
+    for dx in (1.0, np.ones(10)):
+        g = po.general([-10.0, 1.0, 6.0], [-10.0, 1.0, 6.0], dx, 3)
+        gl, gr = g.pad(u)
+        assert math.isclose(gl, -4 / 10, rel_tol=1e-12)
+        assert math.isclose(gr, 125 / 12, rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("dx", [0.1, "vector"])
+@pytest.mark.parametrize("nl", [3, 4, 5])
+def test_general_bc_polynomial_ghost(order, dx, nl):
+    # alpha0 + alpha1 g + alpha2 g' + alpha3 g'' + ... = 0 at the ghost point: the i-th
+    # derivative formula on points 1..order+i is exact to degree order+i-1, so for a
+    # polynomial p of degree order, choosing alpha0 from p makes the left ghost equal p at
+    # the ghost point (x0 = 0, u at x = h, 2h, ...)
+    rng = np.random.default_rng(order * 10 + nl)
+    n = 12
+    h = 0.1
+    xs = h * np.arange(1, n + 1)
+    p = np.polynomial.Polynomial(rng.standard_normal(order + 1))
+    coef = [0.7, -1.3, 0.4, 0.9][: nl - 1]
+    alpha0 = -sum(cf * p.deriv(k)(0.0) if k else cf * p(0.0) for k, cf in enumerate(coef))
+    alpha = [alpha0] + coef
+    dxa = h if dx == 0.1 else np.full(n + 1, h)
+    g = po.general(alpha, alpha, dxa, order)
+    gl, _ = g.pad(p(xs))
+    assert math.isclose(gl, p(0.0), rel_tol=1e-8, abs_tol=1e-8)
+
+
+def test_general_bc_planted_mistake_is_caught():
+    # a wrong sign on b_l must break the Robin equivalence
+    g = po.general([-10.0, 1.0, 6.0], [-10.0, 1.0, 6.0], 1.0, 3)
+    g.b_l = -g.b_l
+    gl, _ = g.pad(np.arange(1.0, 11.0))
+    assert not math.isclose(gl, -4 / 10, rel_tol=1e-6)
