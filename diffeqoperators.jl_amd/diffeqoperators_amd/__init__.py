@@ -17,7 +17,7 @@ from . import _abi
 from .apply import Plan, julia_size, mul_, plan_for_shape, plans_for
 from .batch import Batch, RK4
 from .serve import Server
-from .bcs import (AffineBC, ComposedMultiDimBC, Dirichlet0BC, DirichletBC, GeneralBC,
+from .bcs import (AffineBC, BoundaryPadded, ComposedMultiDimBC, Dirichlet0BC, DirichletBC, GeneralBC,
                   MultiDimBC, MultiDimDirectionalBC, Neumann0BC, NeumannBC, PeriodicBC,
                   RobinBC, compose, decompose)
 from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
@@ -30,7 +30,7 @@ __all__ = [
     "CenteredDifference", "UpwindDifference", "DerivativeOperator", "calculate_weights",
     "RobinBC", "DirichletBC", "Dirichlet0BC", "NeumannBC", "Neumann0BC", "MultiDimBC",
     "MultiDimDirectionalBC", "ComposedMultiDimBC", "AffineBC", "PeriodicBC", "GeneralBC",
-    "compose", "decompose", "GhostDerivativeOperator", "DiffEqOperatorCombination",
+    "compose", "decompose", "BoundaryPadded", "GhostDerivativeOperator", "DiffEqOperatorCombination",
     "mul_", "mul", "plans_for", "plan_for_shape", "Plan", "Batch", "RK4", "Server",
     "julia_size",
 ]

@@ -312,6 +312,10 @@ def combination_times(A: DiffEqOperatorCombination, x):
 
 def mul_(du, A, u, overwrite=True, accum=_abi.ACC_AUTO):
     """mul!(du, A, u): writes A*u into du (adds into du when overwrite=False)."""
+    from .bcs import BoundaryPadded
+    if isinstance(u, BoundaryPadded):
+        # mul!(du, L, Q*u) = mul!(du, L*Q, u)
+        return mul_(du, A * u.Q, u.u, overwrite=overwrite, accum=accum)
     _check_device_tensor(du, "du")
     _check_device_tensor(u, "u")
     accumulate = not overwrite

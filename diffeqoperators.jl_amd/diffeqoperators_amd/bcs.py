@@ -33,6 +33,10 @@ class AffineBC:
     def __repr__(self):
         return f"AffineBC(a_l={self.a_l.tolist()}, b_l={self.b_l}, a_r={self.a_r.tolist()}, b_r={self.b_r})"
 
+    def __mul__(self, u):
+        # Q * u (bc_operators.jl:188-192, multi_dim_bc_operators.jl:210-252): lazy padding
+        return _bc_times(self, u)
+
 
 class PeriodicAtomicBC:
     """PeriodicBC(T) (bc_operators.jl:17-19)."""
@@ -44,6 +48,10 @@ class PeriodicAtomicBC:
 
     def __repr__(self):
         return "PeriodicBC()"
+
+    def __mul__(self, u):
+        # Q * u (bc_operators.jl:188-192, multi_dim_bc_operators.jl:210-252): lazy padding
+        return _bc_times(self, u)
 
 
 def _robin_atomic(l, r, dx, order=1):
@@ -91,6 +99,10 @@ class MultiDimDirectionalBC:
     def __add__(self, other):
         return compose(self, other)
 
+    def __mul__(self, u):
+        # Q * u (bc_operators.jl:188-192, multi_dim_bc_operators.jl:210-252): lazy padding
+        return _bc_times(self, u)
+
 
 class ComposedMultiDimBC:
     """One directional BC per axis (multi_dim_bc_operators.jl:59-61)."""
@@ -101,6 +113,10 @@ class ComposedMultiDimBC:
     @property
     def s(self):
         return self.bcs[0].s
+
+    def __mul__(self, u):
+        # Q * u (bc_operators.jl:188-192, multi_dim_bc_operators.jl:210-252): lazy padding
+        return _bc_times(self, u)
 
 
 def compose(*BCs):
@@ -118,6 +134,57 @@ def compose(*BCs):
 
 def decompose(Q: ComposedMultiDimBC):
     return Q.bcs
+
+
+class BoundaryPadded:
+    """Q * u: u with Q's ghosts, kept lazy like BoundaryPaddedVector / BoundaryPaddedArray /
+    ComposedBoundaryPaddedArray (boundary_padded_arrays.jl:14-180).  L * (Q*u) and
+    mul_(du, L, Q*u) run the same fused kernel as (L*Q) * u; materialize() builds the padded
+    tensor (size + 2 along each padded axis, zero corners for a composed BC)."""
+
+    __array_ufunc__ = None
+
+    def __init__(self, Q, u):
+        self.Q, self.u = Q, u
+
+    def materialize(self):
+        """The padded tensor (Array(Q*u)), assembled with torch from the ghost formulas."""
+        import torch
+        from .apply import _PeriodicPad, _bcs_of, julia_size
+        u = self.u
+        js = julia_size(u)
+        nd = len(js)
+        bcs, _ = _bcs_of(self.Q, nd, js)
+        pshape = list(u.shape)
+        for ax in bcs:
+            pshape[nd - 1 - ax] += 2
+        out = torch.zeros(pshape, dtype=u.dtype, device=u.device)
+        inner = tuple(slice(1, s - 1) if (nd - 1 - k) in bcs else slice(None)
+                      for k, s in enumerate(pshape))
+        out[inner] = u
+        for ax, bc in bcs.items():
+            dim = nd - 1 - ax
+            n = u.shape[dim]
+            x = u.to(torch.float64).movedim(dim, -1)
+            if isinstance(bc, _PeriodicPad):
+                gl, gr = (x[..., n - 1], x[..., 0]) if bc.kind == 1 else (x[..., 0], x[..., n - 1])
+            else:
+                al = torch.as_tensor(bc.a_l, dtype=torch.float64, device=u.device)
+                ar = torch.as_tensor(bc.a_r, dtype=torch.float64, device=u.device)
+                gl = (x[..., : len(bc.a_l)] * al).sum(-1) + bc.b_l
+                gr = (x[..., n - len(bc.a_r):] * ar).sum(-1) + bc.b_r
+            # ghost planes sit inside the other padded axes (zero corners)
+            lo = [slice(1, s - 1) if ((nd - 1 - k) in bcs and k != dim) else slice(None)
+                  for k, s in enumerate(pshape)]
+            hi = list(lo)
+            lo[dim], hi[dim] = 0, pshape[dim] - 1
+            out[tuple(lo)] = gl.to(u.dtype)
+            out[tuple(hi)] = gr.to(u.dtype)
+        return out
+
+
+def _bc_times(Q, u):
+    return BoundaryPadded(Q, u)
 
 
 def is_bc(x) -> bool:

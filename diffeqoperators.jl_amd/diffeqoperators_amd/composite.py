@@ -70,7 +70,9 @@ class DiffEqOperatorCombination:
         return NotImplemented
 
     def __mul__(self, other):
-        from .bcs import is_bc
+        from .bcs import BoundaryPadded, is_bc
+        if isinstance(other, BoundaryPadded):
+            return (self * other.Q) * other.u
         if is_bc(other):
             # *(L::AbstractDiffEqCompositeOperator, Q) = sum(op * Q) (ghost_derivative_operator.jl:11-13)
             return DiffEqOperatorCombination(tuple(op * other for op in self.ops))

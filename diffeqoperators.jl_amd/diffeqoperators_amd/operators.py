@@ -106,10 +106,13 @@ class DerivativeOperator:
         return self + (-other)
 
     def __mul__(self, other):
-        from .bcs import is_bc
+        from .bcs import BoundaryPadded, is_bc
         from .composite import GhostDerivativeOperator
         if is_bc(other):
             return GhostDerivativeOperator(self, other)   # ghost_derivative_operator.jl:7-9
+        if isinstance(other, BoundaryPadded):
+            # L * (Q*u): what mul!(out, L*Q, u) evaluates (ghost_derivative_operator.jl:15-24)
+            return GhostDerivativeOperator(self, other.Q) * other.u
         from .apply import derivative_times_array
         return derivative_times_array(self, other)
 

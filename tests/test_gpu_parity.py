@@ -319,3 +319,52 @@ def test_general_bc_2d():
     got = host64(A * to_dev(u))
     ref = po.ghost_apply([oop(L) for L in ops], [obc(atom), obc(atom)], u)
     assert rel(got, ref) < 1e-12
+
+
+# ---------------------------------------------------------------------------------------
+# Q*u as a lazy padded object (boundary_padded_arrays.jl:14-180): L*(Q*u) is the fused ghost
+# apply; materialize() is the padded array, whose plain padded apply agrees
+
+@pytest.mark.parametrize("shape", [(200,), (90, 70), (20, 24, 28)])
+def test_boundary_padded_lazy_and_materialized(shape):
+    rng = np.random.default_rng(41)
+    js = tuple(reversed(shape))
+    nd = len(shape)
+    atom = D.RobinBC((1.0, 0.5, 0.3), (1.0, 0.5, -0.2), 0.01, 2)
+    if nd == 1:
+        Q = atom
+        L = D.CenteredDifference(2, 4, 0.01, js[0])
+        ops = [L]
+    else:
+        Q = D.compose(*D.MultiDimBC(atom, js))
+        ops = [D.CenteredDifference[N](2, 4, 0.01, js[N - 1]) for N in range(1, nd + 1)]
+    A = ops[0]
+    for op in ops[1:]:
+        A = A + op
+    u = to_dev(rng.uniform(-1, 1, shape))
+    print(f"case: Q*u lazy/materialized {shape}", flush=True)
+    lazy = A * (Q * u)
+    fused = (A * Q) * u
+    assert torch.equal(lazy, fused)
+    du = torch.empty_like(u)
+    D.mul_(du, A * Q, u)
+    assert torch.equal(du, fused)
+    # the materialized padded array: ghosts as the oracle pads each line
+    M = (Q * u).materialize()
+    Mh = host64(M)
+    uh = host64(u)
+    ob = obc(atom)
+    line = np.moveaxis(uh, -1, 0).reshape(js[0], -1).T[0]
+    gl, gr = ob.pad(line)
+    first = np.moveaxis(Mh, -1, 0)
+    sl = (0,) + tuple(1 for _ in range(nd - 1))
+    sr = (js[0] + 1,) + tuple(1 for _ in range(nd - 1))
+    assert abs(first[sl] - gl) < 1e-14 and abs(first[sr] - gr) < 1e-14
+    if nd == 1:
+        # 1-D padded apply (line kernels) against the ghost apply (BPV kernels): the same
+        # numbers for constant coefficients
+        assert rel(host64(L * M), host64(fused)) < 1e-13
+    else:
+        # corners of a composed padding are zero (boundary_padded_arrays.jl:158-180)
+        corner = tuple(0 for _ in range(nd))
+        assert Mh[corner] == 0.0
