@@ -2,7 +2,8 @@
 
 The constructors and composition rules mirror the Julia package (CenteredDifference,
 UpwindDifference, RobinBC / DirichletBC / NeumannBC, compose, L*Q, L1+L2, mul!); applying
-an operator runs hand-written gfx950 HIP kernels in libdop.so (C ABI: include/dop.h).
+an operator runs hand-written gfx950 HIP kernels in libdop.so (C ABI: include/dop.h), and so do
+the concretizations `Array` / `sparse` and the direct solve `ldiv` (Julia's `A \\ u`).
 
     import torch
     from diffeqoperators_amd import CenteredDifference, Dirichlet0BC
@@ -21,6 +22,7 @@ from .bcs import (AffineBC, BoundaryPadded, ComposedMultiDimBC, Dirichlet0BC, Di
                   MultiDimBC, MultiDimDirectionalBC, Neumann0BC, NeumannBC, PeriodicBC,
                   RobinBC, compose, decompose)
 from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
+from .concretize import Array, Solver, ldiv, sparse
 from .fornberg import calculate_weights
 from .operators import CenteredDifference, DerivativeOperator, UpwindDifference
 
@@ -32,7 +34,7 @@ __all__ = [
     "MultiDimDirectionalBC", "ComposedMultiDimBC", "AffineBC", "PeriodicBC", "GeneralBC",
     "compose", "decompose", "BoundaryPadded", "GhostDerivativeOperator", "DiffEqOperatorCombination",
     "mul_", "mul", "plans_for", "plan_for_shape", "Plan", "Batch", "RK4", "Server",
-    "julia_size",
+    "julia_size", "Array", "sparse", "ldiv", "Solver",
 ]
 
 

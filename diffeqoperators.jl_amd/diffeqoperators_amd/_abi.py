@@ -13,7 +13,7 @@ LIB_PATH = os.environ.get("DOP_LIBRARY", os.path.join(_PKG_ROOT, "lib", "libdop.
 
 ABI_VERSION = 1
 
-OK, EINVAL, EDIM, EHIP, ENOMEM, EUNSUPPORTED, EBUDGET = 0, -1, -2, -3, -4, -5, -6
+OK, EINVAL, EDIM, EHIP, ENOMEM, EUNSUPPORTED, EBUDGET, ESINGULAR = 0, -1, -2, -3, -4, -5, -6, -7
 F64, F32, BF16 = 0, 1, 2
 ACC_AUTO, ACC_F64, ACC_F32, ACC_F32_KAHAN, ACC_F32_WIDE = 0, 1, 2, 3, 4
 INPUT_GHOST, INPUT_PADDED = 0, 1
@@ -104,6 +104,14 @@ def lib():
     L.dop_server_stats.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.dop_server_destroy.argtypes = [vp]
     L.dop_server_destroy.restype = None
+    i64p = C.POINTER(C.c_int64)
+    L.dop_plan_csr_rows.argtypes = [vp, vp, i64p, i64p, i64p, vp]
+    L.dop_plan_csr_fill.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.dop_solver_create.argtypes = [vp, vp, C.POINTER(vp)]
+    L.dop_solver_solve.argtypes = [vp, vp, vp, i, vp]
+    L.dop_solver_band.argtypes = [vp, i64p, i64p, i64p]
+    L.dop_solver_destroy.argtypes = [vp]
+    L.dop_solver_destroy.restype = None
     if L.dop_version() != ABI_VERSION:
         raise RuntimeError(f"libdop ABI {L.dop_version()} != {ABI_VERSION}")
     _lib = L

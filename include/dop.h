@@ -43,6 +43,7 @@ extern "C" {
 #define DOP_ENOMEM (-4)       /* device allocation failed */
 #define DOP_EUNSUPPORTED (-5) /* valid in the reference, not provided by this library */
 #define DOP_EBUDGET (-6)      /* request exceeds the device-memory budget */
+#define DOP_ESINGULAR (-7)    /* zero pivot in A \\ u: LinearAlgebra.SingularException */
 
 /* storage types of u / du */
 #define DOP_F64 0
@@ -217,6 +218,43 @@ int dop_server_times(const dop_server* srv, int64_t id, double* release, double*
 int dop_server_stats(const dop_server* srv, int* peak_active, int64_t* peak_bytes,
                      int64_t* iterations);
 void dop_server_destroy(dop_server* srv);
+
+/* ---- concretization (concretization.jl) ----
+ * The map a plan applies, du = A * vec(src) + b (accumulate = 0; src = u for GHOST input, the
+ * padded M for PADDED input), as a CSR matrix with FP64 values: n_rows =
+ * dop_plan_output_elems, n_cols = dop_plan_input_elems, rows and columns in column-major
+ * linear order, columns ascending within a row, exact zeros not stored.
+ * Replaces, for the operator a plan was built from:
+ *   Array / sparse / BandedMatrix of a DerivativeOperator on a line (concretization.jl:8-139,
+ *     upwind :556-810) and its N-D Kron form on Mshape (:425-505)  -- PADDED plans, b = 0;
+ *   Array / sparse of a GhostDerivativeOperator, (L*Q_L, L*Q_b) (:866-913), for 1-D and for
+ *     an operator with a MultiDimBC along its own axis  -- GHOST plans;
+ *   Array of a DiffEqOperatorCombination, the sum of its operators' matrices (:918-924).
+ * GHOST plans of sums with BCs on several axes (which the reference cannot concretize: its
+ * per-operator matrices differ in shape) export the matrix of the same fused apply.
+ * Step 1: row_ptr (device, n_rows + 1 int64) receives the row pointers, *nnz (host) the
+ *   number of entries, *kl / *ku (host, may be NULL) the bandwidths max(row - col) and
+ *   max(col - row).  Synchronizes the stream.
+ * Step 2: col (device int64[nnz]), val (device double[nnz]) and b (device double[n_rows], may
+ *   be NULL) are filled on the stream. */
+int dop_plan_csr_rows(const dop_plan* plan, int64_t* row_ptr, int64_t* nnz, int64_t* kl,
+                      int64_t* ku, void* stream);
+int dop_plan_csr_fill(const dop_plan* plan, const int64_t* row_ptr, int64_t* col, double* val,
+                      double* b, void* stream);
+
+/* ---- direct solve: A \\ u = A_l \\ (u - A_b) (ghost_derivative_operator.jl:39-58) ----
+ * dop_solver_create concretizes a square GHOST plan on the device and factors its band with
+ * partial pivoting (LU, the LAPACK gbtf2 algorithm) once; DOP_ESINGULAR on a zero pivot (the
+ * reference's sparse LU raises SingularException), DOP_EUNSUPPORTED when the band with its
+ * fill-in exceeds 2^30 entries.  dop_solver_solve solves nrhs right-hand sides (columns of n
+ * contiguous FP64 values each, device memory) into x, one block per right-hand side.
+ * Arithmetic is FP64 whatever the plan's storage type.  The reference factors with a general
+ * sparse LU (UMFPACK); both are backward stable, so solutions agree to about cond(A) * eps. */
+typedef struct dop_solver dop_solver;
+int dop_solver_create(const dop_plan* plan, void* stream, dop_solver** out);
+int dop_solver_solve(const dop_solver* solver, void* x, const void* rhs, int nrhs, void* stream);
+int dop_solver_band(const dop_solver* solver, int64_t* n, int64_t* kl, int64_t* ku);
+void dop_solver_destroy(dop_solver* solver);
 
 /* ---- tile autotuner (no counterpart in the reference) ----
  * Times the candidate tile shapes of the plan's fused kernel on scratch buffers and keeps

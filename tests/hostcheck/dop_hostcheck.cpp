@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../diffeqoperators.jl_amd/csrc/dop_build.hpp"
+#include "../../diffeqoperators.jl_amd/csrc/dop_concrete.hpp"
 #include "../../diffeqoperators.jl_amd/csrc/dop_eval.hpp"
 
 namespace {
@@ -24,6 +25,22 @@ void run(const dop::PlanDev& P, T* du, const T* src, int accumulate)
         if (accumulate) v += (double)dop::Store<T>::ld(du, p);
         dop::Store<T>::st(du, p, v);
     }
+}
+
+// the host plan with its table pointers bound to the host copies
+dop::PlanDev bind(const dop::HostPlan& H)
+{
+    dop::PlanDev P = H.meta;
+    for (int a = 0; a < 3; ++a) {
+        dop::AxisDev& X = P.ax[a];
+        const dop::AxisTables& T = H.axis[a];
+        X.w64 = T.w64.empty() ? nullptr : T.w64.data();
+        X.w32 = T.w32.empty() ? nullptr : T.w32.data();
+        X.bJ0 = T.bJ0.empty() ? nullptr : T.bJ0.data();
+        X.bn = T.bn.empty() ? nullptr : T.bn.data();
+        X.bw = T.bw.empty() ? nullptr : T.bw.data();
+    }
+    return P;
 }
 }  // namespace
 
@@ -51,16 +68,7 @@ int hc_apply(const dop_desc* d, void* du, const void* u, int accumulate)
     dop::HostPlan H;
     const int rc = dop::build_host_plan(*d, &H, &g_err);
     if (rc != DOP_OK) return rc;
-    dop::PlanDev P = H.meta;
-    for (int a = 0; a < 3; ++a) {
-        dop::AxisDev& X = P.ax[a];
-        const dop::AxisTables& T = H.axis[a];
-        X.w64 = T.w64.empty() ? nullptr : T.w64.data();
-        X.w32 = T.w32.empty() ? nullptr : T.w32.data();
-        X.bJ0 = T.bJ0.empty() ? nullptr : T.bJ0.data();
-        X.bn = T.bn.empty() ? nullptr : T.bn.data();
-        X.bw = T.bw.empty() ? nullptr : T.bw.data();
-    }
+    dop::PlanDev P = bind(H);
     using namespace dop;
     switch (P.storage) {
     case kF64: run<double, kAccF64>(P, (double*)du, (const double*)u, accumulate); break;
@@ -91,5 +99,37 @@ int64_t hc_sizeof(int which)
     case 2: return (int64_t)sizeof(dop_desc);
     }
     return -1;
+}
+
+// The CSR form of the plan's map (same enumeration and merge as the device export):
+// row_ptr[n_rows+1]; col/val may be NULL (count only, *nnz set); b[n_rows] may be NULL.
+int hc_csr(const dop_desc* d, int64_t* row_ptr, int64_t* col, double* val, double* b,
+           int64_t* nnz)
+{
+    dop::HostPlan H;
+    const int rc = dop::build_host_plan(*d, &H, &g_err);
+    if (rc != DOP_OK) return rc;
+    const dop::PlanDev P = bind(H);
+    if (dop::row_entry_bound(P) > 128) return DOP_EUNSUPPORTED;
+    int64_t o = 0;
+    row_ptr[0] = 0;
+    for (int64_t p = 0; p < P.total; ++p) {
+        int64_t idx[3];
+        dop::point_index(P, p, idx);
+        dop::RowList<128> L;
+        dop::row_entries(P, idx, L);
+        for (int i = 0; i < L.n; ++i) {
+            if (L.val[i] == 0.0) continue;
+            if (col) {
+                col[o] = L.col[i];
+                val[o] = L.val[i];
+            }
+            ++o;
+        }
+        row_ptr[p + 1] = o;
+        if (b) b[p] = L.b;
+    }
+    *nnz = o;
+    return DOP_OK;
 }
 }
