@@ -298,6 +298,231 @@ __global__ __launch_bounds__(NT) void k_gbtrs(int64_t n, int64_t kl, int64_t ku,
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// narrow bands (kl + ku <= kWaveBand): one wave per system, the working set in LDS.
+// Step j of the factorization touches only columns j .. j+kv, so those kv+1 columns live in
+// an LDS ring (column c in slot c mod (kv+1)); column j is final after step j and is written
+// back while column j+kv+1 takes its slot.  The device-memory reads (the entering columns,
+// the multipliers, pivots and vector entries of the solve) run kPipe steps ahead in a
+// register queue, and the lanes of the one wave order their LDS traffic with wave-scope
+// fences only, so no step waits for device memory.
+constexpr int kWaveBand = 62;
+constexpr int kPipe = 8;
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double readlane_d(double v, int l)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// lane i <- lane i+1 (DPP wave_shl:1); lane 63 reads 0
+__device__ inline double wave_shl1(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// (a, i) of the lane with the largest a, the smallest i among equals, over the aligned groups
+// of 2 * span lanes (span = the largest power of two <= m, m >= 1): lanes 0 .. m are in lane
+// 0's group
+__device__ inline void wave_argmax_first(double& a, int& i, int64_t m)
+{
+    for (int off = 1; off <= m && off < 64; off <<= 1) {
+        const double oa = __shfl_xor(a, off);
+        const int oi = __shfl_xor(i, off);
+        if (oa > a || (oa == a && oi < i)) {
+            a = oa;
+            i = oi;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_gbtf2_wave(int64_t n, int64_t kl, int64_t ku, int64_t ldab,
+                                                   double* __restrict__ ab,
+                                                   int64_t* __restrict__ ipiv,
+                                                   int64_t* __restrict__ info)
+{
+    extern __shared__ double win[];   // (kv + 1) columns of ldab
+    const int lane = threadIdx.x;
+    const int64_t kv = kl + ku, W = kv + 1;
+    for (int64_t c = 0; c < W && c < n; ++c)
+        for (int64_t r = lane; r < ldab; r += 64) win[c * ldab + r] = ab[c * ldab + r];
+    // queue slot s holds column W + s (then W + s + kPipe, ...): entries lane and lane + 64
+    double q0[kPipe], q1[kPipe];
+#pragma unroll
+    for (int s = 0; s < kPipe; ++s) {
+        const int64_t c = W + s;
+        q0[s] = (c < n && lane < ldab) ? ab[c * ldab + lane] : 0.0;
+        q1[s] = (c < n && lane + 64 < ldab) ? ab[c * ldab + lane + 64] : 0.0;
+    }
+    __syncthreads();
+    int64_t ju = 0, first_zero = 0;
+    for (int64_t jb = 0; jb < n; jb += kPipe) {
+#pragma unroll
+        for (int s = 0; s < kPipe; ++s) {
+            const int64_t j = jb + s;
+            if (j >= n) break;
+            double* cj = win + (j % W) * ldab;
+            const int64_t km = kl < n - 1 - j ? kl : n - 1 - j;
+                const double v = lane <= km ? cj[kv + lane] : 0.0;
+            double a = lane <= km ? fabs(v) : -1.0;
+            int jpi = lane <= km ? lane : 0;
+            wave_argmax_first(a, jpi, km);
+            const int jr = __builtin_amdgcn_readfirstlane(jpi);
+            const int64_t jp = jr <= km ? jr : km;
+            const double piv = readlane_d(v, (int)jp);
+            if (lane == 0) ipiv[j] = j + jp;
+            if (piv != 0.0) {
+                const int64_t reach = j + ku + jp < n - 1 ? j + ku + jp : n - 1;
+                ju = reach > ju ? reach : ju;
+                if (jp != 0) {
+                    for (int64_t c = j + lane; c <= ju; c += 64) {
+                        double* cc = win + (c % W) * ldab;
+                        const double t = cc[kv + j - c];
+                        cc[kv + j - c] = cc[kv + j + jp - c];
+                        cc[kv + j + jp - c] = t;
+                    }
+                    wave_sync();
+                }
+                if (km > 0) {
+                    const double rcp = 1.0 / cj[kv];
+                    if (lane >= 1 && lane <= km) cj[kv + lane] *= rcp;
+                    wave_sync();
+                    const int64_t w = ju - j;
+                    for (int64_t q = lane; q < km * w; q += 64) {
+                        const int64_t i = 1 + q % km;
+                        const int64_t c = j + 1 + q / km;
+                        double* cc = win + (c % W) * ldab;
+                        cc[kv + j + i - c] -= cj[kv + i] * cc[kv + j - c];
+                    }
+                    wave_sync();
+                }
+            } else if (first_zero == 0) {
+                first_zero = j + 1;
+            }
+            // retire column j; column j + W (queued kPipe steps ago) takes its slot
+            for (int64_t r = lane; r < ldab; r += 64) ab[j * ldab + r] = cj[r];
+            wave_sync();
+            if (j + W < n) {
+                if (lane < ldab) cj[lane] = q0[s];
+                if (lane + 64 < ldab) cj[lane + 64] = q1[s];
+            }
+            const int64_t c = j + W + kPipe;
+            q0[s] = (c < n && lane < ldab) ? ab[c * ldab + lane] : 0.0;
+            q1[s] = (c < n && lane + 64 < ldab) ? ab[c * ldab + lane + 64] : 0.0;
+            wave_sync();
+        }
+    }
+    if (lane == 0) info[0] = first_zero;
+}
+
+// gbtrs for narrow bands: one wave per right-hand side.  The live part of the vector sits in
+// registers, lane t holding y[j+t] (forward, t <= kl) or y[j-t] (backward, t <= kv); a step is
+// two lane reads, one FMA per lane and a one-lane DPP shift of the window.
+__global__ __launch_bounds__(64) void k_gbtrs_wave(int64_t n, int64_t kl, int64_t ku, int64_t ldab,
+                                                   const double* __restrict__ ab,
+                                                   const int64_t* __restrict__ ipiv,
+                                                   const double* __restrict__ bvec,
+                                                   const double* __restrict__ rhs,
+                                                   double* __restrict__ x)
+{
+    const int lane = threadIdx.x;
+    const int64_t kv = kl + ku;
+    const double* r = rhs + (int64_t)blockIdx.x * n;
+    double* y = x + (int64_t)blockIdx.x * n;
+    auto src = [&](int64_t i) { return r[i] - (bvec ? bvec[i] : 0.0); };
+    if (kl > 0) {
+        // forward, step j: y[j] <-> y[ipiv[j]], y[j+i] -= L(j+i, j) y[j] (i = 1 .. lm).
+        // queue slot s of step j: ipiv[j] (lane 0), L(j+lane, j), y[j+kl+1] (lane kl)
+        double v = (lane <= kl && lane < n) ? src(lane) : 0.0;
+        int64_t ql[kPipe];
+        double qm[kPipe], qy[kPipe], qb[kPipe];
+        // Every lane loads (clamped, lane-dependent addresses), so that the loads stay vector
+        // loads the queue keeps in flight (a one-lane load of a uniform address becomes a
+        // scalar load, waited for on the spot); the queue holds the raw loaded values, so that
+        // no arithmetic on them waits before the step that uses them.
+        auto fetch = [&](int64_t j, int s) {
+            const bool ok = j < n - 1;
+            const int64_t lm = ok ? (kl < n - 1 - j ? kl : n - 1 - j) : 0;
+            const int64_t jj = ok ? j : 0;
+            const int64_t ip = jj + lane < n ? jj + lane : n - 1;
+            ql[s] = ipiv[ip];
+            qm[s] = (lane >= 1 && lane <= lm) ? ab[jj * ldab + kv + lane] : 0.0;
+            const int64_t iy = jj + 1 + lane < n ? jj + 1 + lane : n - 1;
+            qy[s] = r[iy];
+            qb[s] = bvec ? bvec[iy] : 0.0;
+        };
+#pragma unroll
+        for (int s = 0; s < kPipe; ++s) fetch(s, s);
+        for (int64_t jb = 0; jb < n - 1; jb += kPipe) {
+#pragma unroll
+            for (int s = 0; s < kPipe; ++s) {
+                const int64_t j = jb + s;
+                if (j >= n - 1) break;
+                const int64_t lm = kl < n - 1 - j ? kl : n - 1 - j;
+                const int d = __builtin_amdgcn_readlane((int)ql[s], 0) - (int)j;   // ipiv[j] - j
+                const double m = qm[s];
+                const double ynew = qy[s] - qb[s];
+                fetch(j + kPipe, s);
+                const double y0 = readlane_d(v, 0);
+                double yj = y0;
+                if (d != 0) {
+                    yj = readlane_d(v, d);
+                    v = lane == d ? y0 : v;
+                }
+                if (lane == 0) y[j] = yj;
+                if (lane >= 1 && lane <= lm) v -= m * yj;
+                v = wave_shl1(v);
+                if (lane == kl) v = ynew;   // lane kl loaded y[j+1+kl]
+            }
+        }
+        if (lane == 0) y[n - 1] = v;
+    } else {
+        for (int64_t i = lane; i < n; i += 64) y[i] = src(i);
+    }
+    // the backward sweep reads what the forward sweep stored
+    __syncthreads();
+    // backward, step j: y[j] /= U(j, j), y[j-i] -= U(j-i, j) y[j] (i = 1 .. min(kv, j)).
+    // queue slot s of step j: U(j-lane, j), y[j-kv-1] (lane kv)
+    double v = (lane <= kv && n - 1 - lane >= 0) ? y[n - 1 - lane] : 0.0;
+    double qu[kPipe], qv[kPipe];
+    auto fetchb = [&](int64_t j, int s) {
+        const bool ok = j >= 0;
+        qu[s] = (ok && lane <= kv && lane <= j) ? ab[j * ldab + kv - lane] : 0.0;
+        const int64_t iy = j - 1 - lane > 0 ? j - 1 - lane : 0;   // lane kv: y[j-kv-1]
+        qv[s] = y[iy];
+    };
+#pragma unroll
+    for (int s = 0; s < kPipe; ++s) fetchb(n - 1 - s, s);
+    for (int64_t jb = n - 1; jb >= 0; jb -= kPipe) {
+#pragma unroll
+        for (int s = 0; s < kPipe; ++s) {
+            const int64_t j = jb - s;
+            if (j < 0) break;
+            const double u = qu[s];
+            const double ynew = qv[s];
+            fetchb(j - kPipe, s);
+            const double yj = readlane_d(v, 0) / readlane_d(u, 0);
+            const int64_t m = kv < j ? kv : j;
+            if (lane == 0) y[j] = yj;
+            if (lane >= 1 && lane <= m) v -= u * yj;
+            v = wave_shl1(v);
+            if (lane == kv) v = ynew;
+        }
+    }
+}
+
 int grid_for(int64_t rows)
 {
     int64_t g = (rows + kExpBlock - 1) / kExpBlock;
@@ -452,11 +677,15 @@ int dop_solver_create(const dop_plan* plan, void* stream, dop_solver** out)
     EXP_HIP(hipMemsetAsync(S->ab, 0, sizeof(double) * S->ldab * n, s));
     hipLaunchKernelGGL(dop::k_band_fill, dim3(dop::grid_for(n)), dim3(dop::kExpBlock), 0, s, rp,
                        col, val, n, kl + ku, S->ldab, S->ab);
-    if ((kl + 1) * (kl + ku + 1) >= 4096)
+    if (kl + ku <= dop::kWaveBand)
+        hipLaunchKernelGGL(dop::k_gbtf2_wave, dim3(1), dim3(64),
+                           (unsigned)(sizeof(double) * (kl + ku + 1) * S->ldab), s, n, kl, ku,
+                           S->ldab, S->ab, S->ipiv, info);
+    else if ((kl + 1) * (kl + ku + 1) >= 4096)
         hipLaunchKernelGGL(dop::k_gbtf2<1024>, dim3(1), dim3(1024), 0, s, n, kl, ku, S->ldab,
                            S->ab, S->ipiv, info);
     else
-        hipLaunchKernelGGL(dop::k_gbtf2<64>, dim3(1), dim3(64), 0, s, n, kl, ku, S->ldab, S->ab,
+        hipLaunchKernelGGL(dop::k_gbtf2<256>, dim3(1), dim3(256), 0, s, n, kl, ku, S->ldab, S->ab,
                            S->ipiv, info);
     EXP_HIP(hipGetLastError());
     EXP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -491,11 +720,14 @@ int dop_solver_solve(const dop_solver* S, void* x, const void* rhs, int nrhs, vo
     if (x == rhs) return efail(DOP_EINVAL, "x must not alias rhs");
     if (nrhs == 0) return DOP_OK;
     const hipStream_t s = (hipStream_t)stream;
-    if (S->kl + S->ku + 1 >= 1024)
+    if (S->kl + S->ku <= dop::kWaveBand)
+        hipLaunchKernelGGL(dop::k_gbtrs_wave, dim3(nrhs), dim3(64), 0, s, S->n, S->kl, S->ku,
+                           S->ldab, S->ab, S->ipiv, S->b, (const double*)rhs, (double*)x);
+    else if (S->kl + S->ku + 1 >= 1024)
         hipLaunchKernelGGL(dop::k_gbtrs<1024>, dim3(nrhs), dim3(1024), 0, s, S->n, S->kl, S->ku,
                            S->ldab, S->ab, S->ipiv, S->b, (const double*)rhs, (double*)x);
     else
-        hipLaunchKernelGGL(dop::k_gbtrs<64>, dim3(nrhs), dim3(64), 0, s, S->n, S->kl, S->ku,
+        hipLaunchKernelGGL(dop::k_gbtrs<256>, dim3(nrhs), dim3(256), 0, s, S->n, S->kl, S->ku,
                            S->ldab, S->ab, S->ipiv, S->b, (const double*)rhs, (double*)x);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return efail(DOP_EHIP, std::string("solve: ") + hipGetErrorString(e));
