@@ -152,4 +152,44 @@ end
 # *(A::GhostDerivativeOperator, u)  (ghost_derivative_operator.jl:26-37)
 gpu_mul(A, u) = gpu_mul!(similar(u), A, u)
 
+# ---- concretization and A \ u --------------------------------------------------------------
+# sparse(A, s) of L*Q (concretization.jl:866-913) as device CSR arrays (0-based columns):
+# `alloc(T, n)` allocates a device vector (e.g. `(T, n) -> ROCVector{T}(undef, n)`).
+function gpu_sparse(A, s::NTuple{N, Int}, alloc; stream = C_NULL) where {N}
+    p = get!(PLAN_CACHE, (A, s, Float64)) do
+        ops, Q = ghost_terms(A)
+        Plan(ops, Q, s, Float64)
+    end
+    n = prod(s)
+    rowptr = alloc(Int64, n + 1)
+    nnz, kl, ku = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ccall((:dop_plan_csr_rows, libdop), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ref{Int64}, Ref{Int64}, Ref{Int64}, Ptr{Cvoid}),
+                p.handle, pointer(rowptr), nnz, kl, ku, stream))
+    col, val, b = alloc(Int64, nnz[]), alloc(Float64, nnz[]), alloc(Float64, n)
+    check(ccall((:dop_plan_csr_fill, libdop), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                p.handle, pointer(rowptr), pointer(col), pointer(val), pointer(b), stream))
+    (rowptr, col, val, b)
+end
+
+const SOLVER_CACHE = IdDict{Any, Ptr{Cvoid}}()
+
+# \(A::GhostDerivativeOperator, u)  (ghost_derivative_operator.jl:39-58); u::Float64 device array
+function gpu_ldiv(A, u; stream = C_NULL)
+    h = get!(SOLVER_CACHE, (A, size(u))) do
+        ops, Q = ghost_terms(A)
+        p = Plan(ops, Q, size(u), Float64)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:dop_solver_create, libdop), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                    p.handle, stream, r))
+        r[]
+    end
+    x = similar(u)
+    check(ccall((:dop_solver_solve, libdop), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Cvoid}),
+                h, pointer(x), pointer(u), 1, stream))
+    x
+end
+
 end # module
