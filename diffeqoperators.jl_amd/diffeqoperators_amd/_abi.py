@@ -112,6 +112,13 @@ def lib():
     L.dop_solver_band.argtypes = [vp, i64p, i64p, i64p]
     L.dop_solver_destroy.argtypes = [vp]
     L.dop_solver_destroy.restype = None
+    dbl = C.c_double
+    L.dop_tsit5_create.argtypes = [vp, dbl, vp, dbl, dbl, dbl, dbl, dbl, vp, C.POINTER(vp)]
+    L.dop_tsit5_run.argtypes = [vp, C.c_int64, i, vp]
+    L.dop_tsit5_set_end.argtypes = [vp, dbl, vp]
+    L.dop_tsit5_status.argtypes = [vp, _dp, _dp, i64p, i64p, C.POINTER(i), vp]
+    L.dop_tsit5_destroy.argtypes = [vp]
+    L.dop_tsit5_destroy.restype = None
     if L.dop_version() != ABI_VERSION:
         raise RuntimeError(f"libdop ABI {L.dop_version()} != {ABI_VERSION}")
     _lib = L

@@ -256,6 +256,27 @@ int dop_solver_solve(const dop_solver* solver, void* x, const void* rhs, int nrh
 int dop_solver_band(const dop_solver* solver, int64_t* n, int64_t* kl, int64_t* ku);
 void dop_solver_destroy(dop_solver* solver);
 
+/* ---- adaptive time stepping (no counterpart in the reference: its method-of-lines tests hand
+ * `(u, p, t) -> A*bc*u` to OrdinaryDiffEq's Tsit5, heat_eqn.jl:4-60) ----
+ * Integrates du/dt = scale * (sum_t L_t Q u) from t0 to t1 in place in u (the plan's storage,
+ * f32 or f64; stage arithmetic and state in f64) with the 5(4) Runge-Kutta pair of Tsitouras
+ * (2011), the error norm sqrt(mean((e / (abstol + reltol max(|u|, |unew|)))^2)) and the step
+ * controller of Hairer, Norsett & Wanner's DOPRI5 (PI, beta 0.04, safety 0.9, ratio [0.2, 10]);
+ * steps are clamped to end exactly at t1.  dt0 <= 0 picks the starting step size by Hairer's
+ * rule.  Every attempted step (six stages, six fused applies, the error reduction and the
+ * controller) is one hipGraph replay with no host round trip; dop_tsit5_run replays up to
+ * max_attempts of them, reading the device state every check_every replays, then commits the
+ * last accepted step into u.  dop_tsit5_status: done = 1 at t1, -1 when the step size
+ * collapsed.  dop_tsit5_set_end moves the end time (continuing from the current state). */
+typedef struct dop_tsit5 dop_tsit5;
+int dop_tsit5_create(const dop_plan* plan, double scale, void* u, double t0, double t1,
+                     double dt0, double abstol, double reltol, void* stream, dop_tsit5** out);
+int dop_tsit5_run(dop_tsit5* integ, int64_t max_attempts, int check_every, void* stream);
+int dop_tsit5_set_end(dop_tsit5* integ, double t1, void* stream);
+int dop_tsit5_status(const dop_tsit5* integ, double* t, double* dt, int64_t* naccept,
+                     int64_t* nreject, int* done, void* stream);
+void dop_tsit5_destroy(dop_tsit5* integ);
+
 /* ---- tile autotuner (no counterpart in the reference) ----
  * Times the candidate tile shapes of the plan's fused kernel on scratch buffers and keeps
  * the fastest in the plan (and in a process-wide cache keyed by dtype, window and

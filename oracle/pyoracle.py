@@ -303,3 +303,89 @@ def composite_mul_padded(ops, M) -> np.ndarray:
     _check(lib().orc_composite_mul_padded(len(ops), structs, len(osz), _ll(osz), _ptr(M),
                                           _ptr(out)), "composite")
     return out
+
+
+# ------------------------------------------------------------------------------------------
+# Adaptive Runge-Kutta time stepping (the reference's method-of-lines tests integrate
+# `(u, p, t) -> A*bc*u` with OrdinaryDiffEq's Tsit5, heat_eqn.jl:4-60; OrdinaryDiffEq is an
+# external dependency, absent here and unpinned: no Manifest.toml).  Restated from the
+# published method: the 5(4) pair of Tsitouras (2011, Comput. Math. Appl. 62: 770-775) and the
+# DOPRI5 step-size controller of Hairer, Norsett & Wanner (Solving ODEs I, 2nd ed., II.4).
+
+TSIT5_C = np.array([0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0])
+TSIT5_A = np.array([
+    [0, 0, 0, 0, 0, 0],
+    [0.161, 0, 0, 0, 0, 0],
+    [-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0],
+    [2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0],
+    [5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0],
+    [5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
+     -0.028269050394068383, 0],
+    [0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+     2.324710524099774],
+], dtype=np.float64)
+TSIT5_B = np.append(TSIT5_A[6], 0.0)          # first same as last: b = the last row of A
+TSIT5_BTILDE = np.array([-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                         -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
+                         0.015151515151515152])
+
+
+def _rms(x):
+    return float(np.sqrt(np.mean(np.square(x))))
+
+
+def tsit5_initial_dt(f, u0, f0, t0, t1, atol, rtol):
+    """Starting step size (Hairer, Norsett & Wanner, II.4) with the scaled RMS norm."""
+    sc = atol + rtol * np.abs(u0)
+    d0, d1 = _rms(u0 / sc), _rms(f0 / sc)
+    h0 = 1e-6 if (d0 < 1e-5 or d1 < 1e-5) else 0.01 * d0 / d1
+    h0 = min(h0, t1 - t0)
+    f1 = f(u0 + h0 * f0)
+    d2 = _rms((f1 - f0) / sc) / h0
+    m = max(d1, d2)
+    h1 = max(1e-6, h0 * 1e-3) if m <= 1e-15 else (0.01 / m) ** (1.0 / 5.0)
+    return min(100 * h0, h1)
+
+
+def tsit5(f, u0, t0, t1, dt0=None, atol=1e-6, rtol=1e-3, max_attempts=10 ** 6):
+    """Integrate du/dt = f(u) from t0 to t1 (steps clamped to end at t1).  Returns
+    (u, t, naccept, nreject)."""
+    u = np.array(u0, dtype=np.float64)
+    k = [None] * 7
+    k[0] = f(u)
+    dt = dt0 if (dt0 is not None and dt0 > 0) else tsit5_initial_dt(f, u, k[0], t0, t1, atol, rtol)
+    dt = min(dt, t1 - t0)
+    t, facold, rejected = t0, 1e-4, False
+    na = nr = 0
+    beta, safe = 0.04, 0.9
+    for _ in range(max_attempts):
+        if t1 - t <= 1e-14 * max(abs(t1), 1.0):
+            break
+        for s in range(1, 7):
+            acc = TSIT5_A[s, 0] * k[0]
+            for j in range(1, s):
+                acc = acc + TSIT5_A[s, j] * k[j]
+            k[s] = f(u + dt * acc)
+        unew = u + dt * (TSIT5_A[6, 0] * k[0] + sum(TSIT5_A[6, j] * k[j] for j in range(1, 6)))
+        k[6] = f(unew)
+        e = dt * sum(TSIT5_BTILDE[j] * k[j] for j in range(7))
+        sc = atol + rtol * np.maximum(np.abs(u), np.abs(unew))
+        err = _rms(e / sc)
+        fac11 = err ** (0.2 - beta * 0.75)
+        if err <= 1.0:
+            t += dt
+            na += 1
+            fac = max(0.1, min(5.0, fac11 / facold ** beta / safe))
+            dtn = dt / fac
+            if rejected:
+                dtn = min(dtn, dt)
+            facold = max(err, 1e-4)
+            rejected = False
+            u = unew
+            k[0] = k[6]
+        else:
+            nr += 1
+            dtn = dt / min(5.0, fac11 / safe)
+            rejected = True
+        dt = min(dtn, t1 - t)
+    return u, t, na, nr
