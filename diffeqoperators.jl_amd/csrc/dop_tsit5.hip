@@ -429,17 +429,24 @@ int dop_tsit5_run(dop_tsit5* I, int64_t max_attempts, int check_every, void* str
     const hipStream_t s = (hipStream_t)stream;
     if (check_every < 1) check_every = 64;
     int64_t left = max_attempts;
-    while (left > 0) {
-        const int64_t k = left < check_every ? left : check_every;
+    dop::TsState H;
+    if (hipMemcpyAsync(&H, I->st, sizeof(H), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return tfail(DOP_EHIP, "tsit5: state read failed");
+    while (left > 0 && !H.done) {
+        // replays between two reads of the state: the attempts the current step size still
+        // needs (plus one), at most check_every -- the operator applies of a replay run even
+        // after the end time is reached, so launching far past it would be wasted work
+        const double need = H.dt > 0.0 ? std::ceil((H.t_end - H.t) / H.dt) + 1.0 : 1.0;
+        int64_t k = need < (double)check_every ? (int64_t)need : check_every;
+        if (k < 1) k = 1;
+        if (k > left) k = left;
         for (int64_t i = 0; i < k; ++i)
             if (hipGraphLaunch(I->exec, s) != hipSuccess) return tfail(DOP_EHIP, "tsit5: graph launch failed");
         left -= k;
-        int32_t done = 0;
-        if (hipMemcpyAsync(&done, (char*)I->st + offsetof(dop::TsState, done), sizeof(done),
-                           hipMemcpyDeviceToHost, s) != hipSuccess ||
+        if (hipMemcpyAsync(&H, I->st, sizeof(H), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             return tfail(DOP_EHIP, "tsit5: state read failed");
-        if (done) break;
     }
     // commit an accepted last step (the commit is idempotent)
     const int64_t n = I->plan->d.host.total;

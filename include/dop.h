@@ -265,8 +265,9 @@ void dop_solver_destroy(dop_solver* solver);
  * steps are clamped to end exactly at t1.  dt0 <= 0 picks the starting step size by Hairer's
  * rule.  Every attempted step (six stages, six fused applies, the error reduction and the
  * controller) is one hipGraph replay with no host round trip; dop_tsit5_run replays up to
- * max_attempts of them, reading the device state every check_every replays, then commits the
- * last accepted step into u.  dop_tsit5_status: done = 1 at t1, -1 when the step size
+ * max_attempts of them, reading the device state after at most check_every replays (fewer
+ * when the current step size reaches the end time sooner), then commits the last accepted
+ * step into u.  dop_tsit5_status: done = 1 at t1, -1 when the step size
  * collapsed.  dop_tsit5_set_end moves the end time (continuing from the current state). */
 typedef struct dop_tsit5 dop_tsit5;
 int dop_tsit5_create(const dop_plan* plan, double scale, void* u, double t0, double t1,
