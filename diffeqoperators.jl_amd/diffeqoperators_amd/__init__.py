@@ -24,6 +24,7 @@ from .bcs import (AffineBC, BoundaryPadded, ComposedMultiDimBC, Dirichlet0BC, Di
 from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
 from .concretize import Array, Solver, ldiv, sparse
 from .integrate import Tsit5
+from .vector_calculus import Curl, Divergence, Gradient
 from .fornberg import calculate_weights
 from .operators import CenteredDifference, DerivativeOperator, UpwindDifference
 
@@ -35,7 +36,7 @@ __all__ = [
     "MultiDimDirectionalBC", "ComposedMultiDimBC", "AffineBC", "PeriodicBC", "GeneralBC",
     "compose", "decompose", "BoundaryPadded", "GhostDerivativeOperator", "DiffEqOperatorCombination",
     "mul_", "mul", "plans_for", "plan_for_shape", "Plan", "Batch", "RK4", "Server",
-    "julia_size", "Array", "sparse", "ldiv", "Solver", "Tsit5",
+    "julia_size", "Array", "sparse", "ldiv", "Solver", "Tsit5", "Gradient", "Divergence", "Curl",
 ]
 
 

@@ -389,3 +389,47 @@ def tsit5(f, u0, t0, t1, dt0=None, atol=1e-6, rtol=1e-3, max_attempts=10 ** 6):
             rejected = True
         dt = min(dtn, t1 - t)
     return u, t, na, nr
+
+
+# ------------------------------------------------------------------------------------------
+# Vector calculus (vector_calculus_operators.jl:5-88, vector_calculus_convolutions.jl).  Every
+# derivative is the centered line operator along one axis of a padded array; for 2-D and 3-D
+# arrays the reference's kernels (vector_calculus_convolutions.jl:173-1067) index the
+# transverse axes of the padded input at 1:n, not at the interior 2:n+1 (restated as read);
+# 1-D has no transverse axis.
+
+def vc_line_op(op, M, k, transverse_view):
+    """op (a line operator along Julia axis k, 0-based) applied to every line of the padded
+    array M (numpy order), transverse axes at 0..n-1 (transverse_view) or 1..n."""
+    nd = M.ndim
+    ax = nd - 1 - k
+    Mt = np.moveaxis(np.asarray(M, dtype=np.float64), ax, -1)
+    tshape = tuple(s - 2 for s in Mt.shape[:-1])
+    off = 0 if transverse_view else 1
+    out = np.zeros(tshape + (Mt.shape[-1] - 2,))
+    for idx in np.ndindex(*tshape):
+        out[idx] = op.mul_padded(np.ascontiguousarray(Mt[tuple(i + off for i in idx)]))
+    return np.moveaxis(out, -1, ax)
+
+
+def gradient(ops, M):
+    """Gradient * M: shape (N,) + interior shape (component first in numpy order)."""
+    nd = M.ndim
+    return np.stack([vc_line_op(ops[k], M, k, nd in (2, 3)) for k in range(nd)])
+
+
+def divergence(ops, M):
+    """Divergence * M, M of shape (N,) + padded shape."""
+    nd = M.ndim - 1
+    out = None
+    for k in range(nd):
+        v = vc_line_op(ops[k], M[k], k, nd in (2, 3))
+        out = v if out is None else out + v
+    return out
+
+
+def curl(ops, M):
+    """Curl * M, M of shape (3,) + padded 3-D shape: (d2 u3 - d3 u2, d3 u1 - d1 u3,
+    d1 u2 - d2 u1) with d_k along Julia axis k."""
+    d = lambda k, c: vc_line_op(ops[k], M[c], k, True)   # noqa: E731
+    return np.stack([d(1, 2) - d(2, 1), d(2, 0) - d(0, 2), d(0, 1) - d(1, 0)])
