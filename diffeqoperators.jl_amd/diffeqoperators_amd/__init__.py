@@ -25,6 +25,7 @@ from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
 from .concretize import Array, Solver, ldiv, sparse
 from .integrate import Tsit5
 from .vector_calculus import Curl, Divergence, Gradient
+from .nonlinear import nonlinear_diffusion, nonlinear_diffusion_
 from .fornberg import calculate_weights
 from .operators import CenteredDifference, DerivativeOperator, UpwindDifference
 
@@ -37,6 +38,7 @@ __all__ = [
     "compose", "decompose", "BoundaryPadded", "GhostDerivativeOperator", "DiffEqOperatorCombination",
     "mul_", "mul", "plans_for", "plan_for_shape", "Plan", "Batch", "RK4", "Server",
     "julia_size", "Array", "sparse", "ldiv", "Solver", "Tsit5", "Gradient", "Divergence", "Curl",
+    "nonlinear_diffusion", "nonlinear_diffusion_",
 ]
 
 

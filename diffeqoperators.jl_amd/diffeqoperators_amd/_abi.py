@@ -119,6 +119,7 @@ def lib():
     L.dop_tsit5_status.argtypes = [vp, _dp, _dp, i64p, i64p, C.POINTER(i), vp]
     L.dop_tsit5_destroy.argtypes = [vp]
     L.dop_tsit5_destroy.restype = None
+    L.dop_nonlinear_diffusion.argtypes = [C.POINTER(vp), C.POINTER(vp), i, vp, vp, vp, vp]
     if L.dop_version() != ABI_VERSION:
         raise RuntimeError(f"libdop ABI {L.dop_version()} != {ABI_VERSION}")
     _lib = L

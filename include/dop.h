@@ -256,6 +256,15 @@ int dop_solver_solve(const dop_solver* solver, void* x, const void* rhs, int nrh
 int dop_solver_band(const dop_solver* solver, int64_t* n, int64_t* kl, int64_t* ku);
 void dop_solver_destroy(dop_solver* solver);
 
+/* ---- nonlinear_diffusion! (derivative_operator.jl:34-66) ----
+ * du = sum_{l=0..s} C(s, l) (D^{l+f} q) .* (D^{s-l} p) on padded vectors p, q (length n+2;
+ * s = nterms - 1): dq[l] is the plan of CenteredDifference(l+f, approx, dx, n) on a padded
+ * vector, or NULL for f + l = 0 (then the interior q[2:n+1]); dp[l] likewise for s - l.
+ * All plans are 1-D PADDED plans of one length and storage (f64 or f32; the products and sums
+ * are formed in f64 in the reference's order).  One launch. */
+int dop_nonlinear_diffusion(const dop_plan* const* dq, const dop_plan* const* dp, int nterms,
+                            void* du, const void* p, const void* q, void* stream);
+
 /* ---- adaptive time stepping (no counterpart in the reference: its method-of-lines tests hand
  * `(u, p, t) -> A*bc*u` to OrdinaryDiffEq's Tsit5, heat_eqn.jl:4-60) ----
  * Integrates du/dt = scale * (sum_t L_t Q u) from t0 to t1 in place in u (the plan's storage,

@@ -10,6 +10,7 @@ is a C-contiguous numpy array of shape (n3, n2, n1).
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -433,3 +434,17 @@ def curl(ops, M):
     d1 u2 - d2 u1) with d_k along Julia axis k."""
     d = lambda k, c: vc_line_op(ops[k], M[c], k, True)   # noqa: E731
     return np.stack([d(1, 2) - d(2, 1), d(2, 0) - d(0, 2), d(0, 1) - d(1, 0)])
+
+
+def nonlinear_diffusion(s, f, approx, p, q, dx, n):
+    """nonlinear_diffusion!{N}(du, s, f, approx, p, q, dx, n) (derivative_operator.jl:34-66),
+    restated term by term with the oracle's line kernel: D^k = CenteredDifference(k, approx,
+    dx, n) on the padded vectors, D^0 = the interior."""
+    def D(k, v):
+        if k == 0:
+            return np.asarray(v, dtype=np.float64)[1:n + 1]
+        return Op(0, k, approx, dx, n).mul_padded(np.ascontiguousarray(v, dtype=np.float64))
+    du = D(f, q) * D(s, p)
+    for l in range(1, s):
+        du = du + (math.comb(s, l) * D(l + f, q)) * D(s - l, p)
+    return du + D(f + s, q) * D(0, p)
