@@ -109,6 +109,7 @@ int dop_plan_create(const dop_desc* desc, dop_plan** out)
         if ((r = upload(p, T.bJ0, &X.bJ0)) != DOP_OK) break;
         if ((r = upload(p, T.bn, &X.bn)) != DOP_OK) break;
         if ((r = upload(p, T.bw, &X.bw)) != DOP_OK) break;
+        if ((r = upload(p, T.lbc, &X.lbc)) != DOP_OK) break;
     }
     if (r == DOP_OK) {
         if (hipMalloc((void**)&p->dev, sizeof(dop::PlanDev)) != hipSuccess) {

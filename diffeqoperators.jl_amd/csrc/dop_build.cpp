@@ -146,6 +146,45 @@ int build_host_plan(const dop_desc& D, HostPlan* out, std::string* err)
                 X.bl = q.b_l;
                 X.br = q.b_r;
             }
+            if (const dop_affine_bc* lb = D.line_bc[a]) {
+                // one BC per line: the other axes' extents, column-major
+                if (q.kind != DOP_BC_AFFINE) {
+                    *err = "per-line boundary conditions need an affine axis BC";
+                    return DOP_EINVAL;
+                }
+                int64_t nlines = 1;
+                for (int b = 0; b < D.ndim; ++b) {
+                    X.lstride[b] = b == a ? 0 : nlines;
+                    if (b != a) nlines *= P.n[b];
+                }
+                T.lbc.assign((size_t)nlines * kLineBC, 0.0);
+                int mnl = 0, mnr = 0;
+                for (int64_t l = 0; l < nlines; ++l) {
+                    const dop_affine_bc& e = lb[l];
+                    if (e.kind != DOP_BC_AFFINE || e.n_l < 0 || e.n_r < 0 ||
+                        e.n_l > DOP_MAX_BC_TAPS || e.n_r > DOP_MAX_BC_TAPS || e.n_l > P.n[a] ||
+                        e.n_r > P.n[a]) {
+                        *err = "per-line boundary condition " + std::to_string(l) +
+                               ": not affine, or more taps than the axis has points";
+                        return DOP_EINVAL;
+                    }
+                    double* o = T.lbc.data() + (size_t)l * kLineBC;
+                    o[0] = e.n_l;
+                    o[1] = e.n_r;
+                    for (int k = 0; k < e.n_l; ++k) o[2 + k] = e.a_l[k];
+                    o[2 + kMaxBCTaps] = e.b_l;
+                    for (int k = 0; k < e.n_r; ++k) o[3 + kMaxBCTaps + k] = e.a_r[k];
+                    o[3 + 2 * kMaxBCTaps] = e.b_r;
+                    mnl = std::max(mnl, (int)e.n_l);
+                    mnr = std::max(mnr, (int)e.n_r);
+                }
+                // the widest line bounds the entries of a matrix row (concretization)
+                X.nl = mnl;
+                X.nr = mnr;
+            }
+        } else if (a < D.ndim && D.line_bc[a]) {
+            *err = "per-line boundary conditions need GHOST input and a present axis BC";
+            return DOP_EINVAL;
         }
         int64_t ts = 0;
         if (D.input == DOP_INPUT_PADDED) {

@@ -32,7 +32,7 @@ DOP_HD inline int row_entry_bound(const PlanDev& P)
 // 221-252 for the N-D form).  col < 0 marks a constant term.
 template <typename F>
 DOP_HD inline void ghost_entries(const AxisDev& X, bool left, int64_t base, int64_t S, double w,
-                                 F& emit)
+                                 int64_t line, F& emit)
 {
 #pragma clang fp contract(off)
     if (X.periodic) {
@@ -40,13 +40,18 @@ DOP_HD inline void ghost_entries(const AxisDev& X, bool left, int64_t base, int6
         emit(base + (last ? (int64_t)(X.n - 1) * S : 0), w);
         return;
     }
+    const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
     if (left) {
-        for (int m = 0; m < X.nl; ++m) emit(base + (int64_t)m * S, w * X.al[m]);
-        emit(-1, w * X.bl);
+        const int nl = e ? (int)e[0] : X.nl;
+        const double* al = e ? e + 2 : X.al;
+        for (int m = 0; m < nl; ++m) emit(base + (int64_t)m * S, w * al[m]);
+        emit(-1, w * (e ? e[2 + kMaxBCTaps] : X.bl));
     } else {
-        const int64_t o = (int64_t)X.n - X.nr;
-        for (int m = 0; m < X.nr; ++m) emit(base + (o + m) * S, w * X.ar[m]);
-        emit(-1, w * X.br);
+        const int nr = e ? (int)e[1] : X.nr;
+        const double* ar = e ? e + 3 + kMaxBCTaps : X.ar;
+        const int64_t o = (int64_t)X.n - nr;
+        for (int m = 0; m < nr; ++m) emit(base + (o + m) * S, w * ar[m]);
+        emit(-1, w * (e ? e[3 + 2 * kMaxBCTaps] : X.br));
     }
 }
 
@@ -72,10 +77,13 @@ DOP_HD inline void row_entries(const PlanDev& P, const int64_t* idx, F& emit)
             const int nt = X.bn[b];
             const double* w = X.bw + (int64_t)b * X.KB;
             const int joff = P.ghost ? -1 : 0;
+            const int64_t line = X.lbc ? idx[0] * X.lstride[0] + idx[1] * X.lstride[1] +
+                                             idx[2] * X.lstride[2]
+                                       : 0;
             for (int k = 0; k < nt; ++k) {
                 const int J = J0 + k;
-                if (P.ghost && J == 0) ghost_entries(X, true, base, S, w[k], emit);
-                else if (P.ghost && J == X.n + 1) ghost_entries(X, false, base, S, w[k], emit);
+                if (P.ghost && J == 0) ghost_entries(X, true, base, S, w[k], line, emit);
+                else if (P.ghost && J == X.n + 1) ghost_entries(X, false, base, S, w[k], line, emit);
                 else emit(base + (int64_t)(J + joff) * S, w[k]);
             }
         }

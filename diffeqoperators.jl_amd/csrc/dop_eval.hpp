@@ -128,25 +128,40 @@ struct Acc<kAccF32Wide> {
 };
 
 // ---------------------------------------------------------------------------------------
-// ghost values of one line: a . u[0:nl] + b (bc_operators.jl:188-191)
+// ghost values of one line: a . u[0:nl] + b (bc_operators.jl:188-191), with the line's own
+// BC when the axis has one per line (multi_dim_bc_operators.jl:34, 93-95)
 template <typename T>
-DOP_HD inline double ghost_left(const AxisDev& X, const T* src, int64_t base, int64_t S)
+DOP_HD inline double ghost_left(const AxisDev& X, const T* src, int64_t base, int64_t S,
+                                int64_t line)
 {
     if (X.periodic) return (double)Store<T>::ld(src, base + (X.periodic == 1 ? (int64_t)(X.n - 1) * S : 0));
+    const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
+    const int nl = e ? (int)e[0] : X.nl;
+    const double* al = e ? e + 2 : X.al;
     double g = 0.0;
-    for (int k = 0; k < X.nl; ++k) g = fma(X.al[k], (double)Store<T>::ld(src, base + k * S), g);
-    return g + X.bl;
+    for (int k = 0; k < nl; ++k) g = fma(al[k], (double)Store<T>::ld(src, base + k * S), g);
+    return g + (e ? e[2 + kMaxBCTaps] : X.bl);
 }
 
 template <typename T>
-DOP_HD inline double ghost_right(const AxisDev& X, const T* src, int64_t base, int64_t S)
+DOP_HD inline double ghost_right(const AxisDev& X, const T* src, int64_t base, int64_t S,
+                                 int64_t line)
 {
     if (X.periodic) return (double)Store<T>::ld(src, base + (X.periodic == 2 ? (int64_t)(X.n - 1) * S : 0));
+    const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
+    const int nr = e ? (int)e[1] : X.nr;
+    const double* ar = e ? e + 3 + kMaxBCTaps : X.ar;
     double g = 0.0;
-    const int64_t o = (int64_t)X.n - X.nr;
-    for (int k = 0; k < X.nr; ++k)
-        g = fma(X.ar[k], (double)Store<T>::ld(src, base + (o + k) * S), g);
-    return g + X.br;
+    const int64_t o = (int64_t)X.n - nr;
+    for (int k = 0; k < nr; ++k)
+        g = fma(ar[k], (double)Store<T>::ld(src, base + (o + k) * S), g);
+    return g + (e ? e[3 + 2 * kMaxBCTaps] : X.br);
+}
+
+// line index of output point idx along axis a (per-line BCs only)
+DOP_HD inline int64_t line_of(const AxisDev& X, const int64_t* idx)
+{
+    return X.lbc ? idx[0] * X.lstride[0] + idx[1] * X.lstride[1] + idx[2] * X.lstride[2] : 0;
 }
 
 constexpr int kEvalUnroll = 8;
@@ -155,7 +170,7 @@ constexpr int kEvalUnroll = 8;
 // formed once, and every tap's load is issued before the first FMA.
 template <typename T>
 DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* src, int r,
-                                  int64_t base, int64_t S)
+                                  int64_t base, int64_t S, int64_t line)
 {
     const int b = r < X.rlo ? r : X.rlo + (r - X.rhi);
     const int J0 = X.bJ0[b];
@@ -164,8 +179,8 @@ DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* s
     const int joff = P.ghost ? -1 : 0;
     const bool gl_used = P.ghost && J0 == 0;
     const bool gr_used = P.ghost && J0 + nt - 1 == X.n + 1;
-    const double gl = gl_used ? ghost_left<T>(X, src, base, S) : 0.0;
-    const double gr = gr_used ? ghost_right<T>(X, src, base, S) : 0.0;
+    const double gl = gl_used ? ghost_left<T>(X, src, base, S, line) : 0.0;
+    const double gr = gr_used ? ghost_right<T>(X, src, base, S, line) : 0.0;
     double s = 0.0;
     if (nt <= kEvalUnroll) {
         double v[kEvalUnroll];
@@ -229,7 +244,7 @@ DOP_HD inline double eval_point(const PlanDev& P, const T* src, const int64_t* i
                     acc.add(w[k], (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
             }
         } else {
-            acc.add_sum(boundary_row<T>(P, X, src, r, base, S));
+            acc.add_sum(boundary_row<T>(P, X, src, r, base, S, line_of(X, idx)));
         }
     }
     return acc.value();

@@ -47,7 +47,14 @@ struct AxisDev {
     const int32_t* bJ0;   // [nb] padded start
     const int32_t* bn;    // [nb] taps
     const double* bw;     // [nb][KB]
+    // per-line BCs (nullptr: al/bl/ar/br above): line l at lbc + l * kLineBC, laid out as
+    // nl, nr, al[8], bl, ar[8], br; line of a point = sum over the other axes b of idx[b] *
+    // lstride[b]
+    const double* lbc;
+    int64_t lstride[kMaxDim];
 };
+
+constexpr int kLineBC = 2 * kMaxBCTaps + 4;
 
 struct PlanDev {
     int32_t ndim;

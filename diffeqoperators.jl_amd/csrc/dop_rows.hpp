@@ -33,6 +33,7 @@ struct AxisTables {
     std::vector<float> w32;
     std::vector<int32_t> bJ0, bn;
     std::vector<double> bw;
+    std::vector<double> lbc;   // per-line BCs, kLineBC per line (empty: one BC per axis)
 };
 
 // Merges all operators acting on one axis (sum in the given order) into row programs and

@@ -11,7 +11,7 @@ import os
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("DOP_LIBRARY", os.path.join(_PKG_ROOT, "lib", "libdop.so"))
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK, EINVAL, EDIM, EHIP, ENOMEM, EUNSUPPORTED, EBUDGET, ESINGULAR = 0, -1, -2, -3, -4, -5, -6, -7
 F64, F32, BF16 = 0, 1, 2
@@ -51,6 +51,7 @@ class DopDesc(C.Structure):
         ("semantics", C.c_int32), ("n_ops", C.c_int32), ("size", C.c_int64 * 3),
         ("padded", C.c_int32 * 3), ("transverse_view", C.c_int32),
         ("ops", C.POINTER(DopOperator)), ("bc", DopAffineBC * 3),
+        ("line_bc", C.POINTER(DopAffineBC) * 3),
     ]
 
 

@@ -12,7 +12,7 @@ import ctypes as C
 import torch
 
 from . import _abi
-from .bcs import AffineBC, ComposedMultiDimBC, MultiDimDirectionalBC, PeriodicAtomicBC
+from .bcs import AffineBC, ComposedMultiDimBC, MultiDimDirectionalBC, PeriodicAtomicBC, _LineBCs
 from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
 from .operators import DerivativeOperator
 
@@ -114,6 +114,24 @@ def make_desc(terms, jsize, storage, *, input_kind, semantics, bcs=None, padded=
         if isinstance(bc, _PeriodicPad):
             q.present, q.kind, q.n_l, q.n_r = 1, bc.kind, 0, 0
             continue
+        if isinstance(bc, _LineBCs):
+            # one BC per line: the axis entry only marks the axis as padded
+            q.present, q.kind, q.n_l, q.n_r = 1, _abi.BC_AFFINE, 0, 0
+            flat = bc.flat()
+            arr = (_abi.DopAffineBC * len(flat))()
+            for i, b in enumerate(flat):
+                if len(b.a_l) > _abi.MAX_BC_TAPS or len(b.a_r) > _abi.MAX_BC_TAPS:
+                    raise ValueError("boundary condition order too high")
+                e = arr[i]
+                e.present, e.kind, e.n_l, e.n_r = 1, _abi.BC_AFFINE, len(b.a_l), len(b.a_r)
+                for k, v in enumerate(b.a_l):
+                    e.a_l[k] = float(v)
+                for k, v in enumerate(b.a_r):
+                    e.a_r[k] = float(v)
+                e.b_l, e.b_r = float(b.b_l), float(b.b_r)
+            d.line_bc[ax] = C.cast(arr, C.POINTER(_abi.DopAffineBC))
+            keep.append(arr)
+            continue
         if len(bc.a_l) > _abi.MAX_BC_TAPS or len(bc.a_r) > _abi.MAX_BC_TAPS:
             raise ValueError("boundary condition order too high")
         q.present = 1
@@ -153,8 +171,9 @@ def _bcs_of(Q, ndim, jsize):
     if ndim == 1:
         raise AssertionError("multi-dimensional BCs need arrays of 2 or more dimensions")
     if isinstance(Q, MultiDimDirectionalBC):
-        if tuple(Q.s) != tuple(jsize):
-            raise AssertionError("Size of the BCs array in the MultiDimBC is incorrect")
+        _check_dir_size(Q, jsize, "Size of the BCs array in the MultiDimBC is incorrect")
+        if Q.lines is not None:
+            return {Q.D - 1: _LineBCs(Q.lines)}, _abi.SEM_LINES
         bc = Q.bc
         if isinstance(bc, PeriodicAtomicBC):
             bc = _PeriodicPad(_abi.BC_PERIODIC_ND)
@@ -165,14 +184,27 @@ def _bcs_of(Q, ndim, jsize):
         all_periodic = all(isinstance(b.bc, PeriodicAtomicBC) for b in Q.bcs)
         out = {}
         for b in Q.bcs:
-            if tuple(b.s) != tuple(jsize):
-                raise AssertionError(f"Size of the BCs array for dimension {b.D} is incorrect")
+            _check_dir_size(b, jsize, f"Size of the BCs array for dimension {b.D} is incorrect")
+            if b.lines is not None:
+                out[b.D - 1] = _LineBCs(b.lines)
+                continue
             bc = b.bc
             if isinstance(bc, PeriodicAtomicBC):
                 bc = _PeriodicPad(_abi.BC_PERIODIC_ND if all_periodic else _abi.BC_PERIODIC)
             out[b.D - 1] = bc
         return out, _abi.SEM_LINES
     raise TypeError(f"not a boundary condition: {type(Q)}")
+
+
+def _check_dir_size(Q, jsize, msg):
+    """The BC array of a MultiDimDirectionalBC must match the array's other axes
+    (multi_dim_bc_operators.jl:178-192 / perpindex)."""
+    if Q.lines is not None:
+        other = tuple(v for k, v in enumerate(jsize) if k != Q.D - 1)
+        if len(jsize) != Q.lines.ndim + 1 or tuple(Q.lines.shape) != other:
+            raise AssertionError(msg)
+    elif tuple(Q.s) != tuple(jsize):
+        raise AssertionError(msg)
 
 
 def _ghost_groups(terms, ndim, jsize):

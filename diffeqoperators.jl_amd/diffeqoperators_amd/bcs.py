@@ -91,10 +91,18 @@ def _div(a, b):
 
 class MultiDimDirectionalBC:
     """An atomic BC extended over every line along axis D of an array of size s
-    (multi_dim_bc_operators.jl:54-57, 97-101)."""
+    (multi_dim_bc_operators.jl:54-57, 97-101), or one atomic BC per line: `lines` is an array
+    (numpy, dtype object) of AffineBC indexed like the array's other axes in Julia order
+    (MultiDimBC{dim}(BC::Array), :93-95; line I gets BC[I], :34); then s is None."""
 
-    def __init__(self, bc: AffineBC, D: int, s):
-        self.bc, self.D, self.s = bc, int(D), tuple(int(v) for v in s)
+    def __init__(self, bc: AffineBC, D: int, s, lines=None):
+        self.bc, self.D = bc, int(D)
+        self.s = None if s is None else tuple(int(v) for v in s)
+        self.lines = lines
+
+    @property
+    def ndims(self):
+        return len(self.s) if self.s is not None else self.lines.ndim + 1
 
     def __add__(self, other):
         return compose(self, other)
@@ -114,6 +122,10 @@ class ComposedMultiDimBC:
     def s(self):
         return self.bcs[0].s
 
+    @property
+    def ndims(self):
+        return self.bcs[0].ndims
+
     def __mul__(self, u):
         # Q * u (bc_operators.jl:188-192, multi_dim_bc_operators.jl:210-252): lazy padding
         return _bc_times(self, u)
@@ -121,7 +133,7 @@ class ComposedMultiDimBC:
 
 def compose(*BCs):
     """multi_dim_bc_operators.jl:178-192."""
-    N = len(BCs[0].s)
+    N = BCs[0].ndims
     Ds = [b.D for b in BCs]
     if len(BCs) != N:
         raise ValueError("ArgumentError: There must be enough BCs to cover every dimension - "
@@ -168,6 +180,14 @@ class BoundaryPadded:
             x = u.to(torch.float64).movedim(dim, -1)
             if isinstance(bc, _PeriodicPad):
                 gl, gr = (x[..., n - 1], x[..., 0]) if bc.kind == 1 else (x[..., 0], x[..., n - 1])
+            elif isinstance(bc, _LineBCs):
+                al, bl, ar, br = (torch.as_tensor(t, dtype=torch.float64, device=u.device)
+                                  for t in bc.tables())
+                xl = x.reshape(-1, n)   # C order over the other axes = Julia's column-major lines
+                k = al.shape[1]
+                gl = ((xl[:, :k] * al).sum(-1) + bl).reshape(x.shape[:-1])
+                k = ar.shape[1]
+                gr = ((xl[:, n - k:] * ar).sum(-1) + br).reshape(x.shape[:-1])
             else:
                 al = torch.as_tensor(bc.a_l, dtype=torch.float64, device=u.device)
                 ar = torch.as_tensor(bc.a_r, dtype=torch.float64, device=u.device)
@@ -181,6 +201,32 @@ class BoundaryPadded:
             out[tuple(lo)] = gl.to(u.dtype)
             out[tuple(hi)] = gr.to(u.dtype)
         return out
+
+
+class _LineBCs:
+    """One affine BC per line along an axis (lines in Julia column-major order)."""
+
+    def __init__(self, lines):
+        self.lines = lines
+
+    def key(self):
+        return ("lines", id(self.lines))
+
+    def flat(self):
+        return list(self.lines.flatten(order="F"))
+
+    def tables(self):
+        """(a_l, b_l, a_r, b_r) per line, taps right-aligned for a_r and zero-padded."""
+        bcs = self.flat()
+        kl = max(len(b.a_l) for b in bcs)
+        kr = max(len(b.a_r) for b in bcs)
+        al = np.zeros((len(bcs), kl))
+        ar = np.zeros((len(bcs), kr))
+        for i, b in enumerate(bcs):
+            al[i, : len(b.a_l)] = b.a_l
+            if len(b.a_r):
+                ar[i, kr - len(b.a_r):] = b.a_r
+        return al, np.array([b.b_l for b in bcs]), ar, np.array([b.b_r for b in bcs])
 
 
 def _bc_times(Q, u):
@@ -267,7 +313,17 @@ def _multidim(bc, s):
     return tuple(MultiDimDirectionalBC(bc, dim + 1, s) for dim in range(len(s)))
 
 
-def _multidim_dim(dim, bc, s):
+def _multidim_dim(dim, bc, s=None):
+    if s is None:
+        # MultiDimBC{dim}(BC::Array{<:AtomicBC}) (multi_dim_bc_operators.jl:93-95)
+        lines = np.asarray(bc, dtype=object)
+        if lines.ndim < 1 or lines.ndim > 2:
+            raise ValueError("the BC array must have 1 or 2 dimensions (2-D or 3-D arrays)")
+        for b in lines.flat:
+            if not isinstance(b, AffineBC):
+                raise TypeError("per-line boundary conditions must be affine (Robin / Dirichlet / "
+                                "Neumann / GeneralBC)")
+        return MultiDimDirectionalBC(lines.flat[0], dim, None, lines=lines)
     return MultiDimDirectionalBC(bc, dim, s)
 
 

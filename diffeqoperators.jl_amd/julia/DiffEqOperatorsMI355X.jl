@@ -35,6 +35,7 @@ struct DopDesc
     ndim::Int32; storage::Int32; accum::Int32; input::Int32; semantics::Int32; n_ops::Int32
     size::NTuple{3, Int64}; padded::NTuple{3, Int32}; transverse_view::Int32
     ops::Ptr{DopOperator}; bc::NTuple{3, DopAffineBC}
+    line_bc::NTuple{3, Ptr{DopAffineBC}}   # per-line BCs of an axis (C_NULL: bc[a] on all lines)
 end
 
 const DOP_INPUT_GHOST, DOP_INPUT_PADDED = 0, 1
@@ -96,6 +97,14 @@ axis_bcs(Q::ComposedMultiDimBC{T, B}, s) where {T, B} =
     Dict(d => (first(Q.BCs[d]), B <: PeriodicBC ? DOP_BC_PERIODIC_ND : DOP_BC_PERIODIC)
          for d in 1:length(Q.BCs))
 
+# per-line BCs: an axis whose MultiDimDirectionalBC holds different atomic BCs on its lines
+# (MultiDimBC{dim}(BC::Array), multi_dim_bc_operators.jl:93-95) passes all of them, in the
+# array's column-major order, as line_bc (affine BCs only)
+line_bcs(Q::MultiDimDirectionalBC{T, B, D}) where {T, B, D} =
+    all(b -> b === first(Q.BCs), Q.BCs) ? Dict() : Dict(D => [dop_bc(b) for b in vec(Q.BCs)])
+line_bcs(Q::ComposedMultiDimBC) = merge(Dict(), (line_bcs(q) for q in Q.BCs)...)
+line_bcs(Q) = Dict()
+
 mutable struct Plan
     handle::Ptr{Cvoid}
     keep::Vector{Any}
@@ -108,9 +117,12 @@ function Plan(ops::Vector, Q, sz::NTuple{N, Int}, ::Type{T}; input = DOP_INPUT_G
     push!(keep, dops)
     bcs = Q === nothing ? Dict() : axis_bcs(Q, sz)
     sem = (N == 1 && Q isa AtomicBC) ? DOP_SEM_BPV : DOP_SEM_LINES
+    lines = Q === nothing ? Dict() : line_bcs(Q)
+    push!(keep, lines)
     desc = DopDesc(N, storage_code(T), 0, input, sem, length(dops),
                    ntuple(i -> i <= N ? sz[i] : 1, 3), Int32.(padded), transverse_view,
-                   pointer(dops), ntuple(i -> haskey(bcs, i) ? dop_bc(bcs[i]...) : NO_BC, 3))
+                   pointer(dops), ntuple(i -> haskey(bcs, i) ? dop_bc(bcs[i]...) : NO_BC, 3),
+                   ntuple(i -> haskey(lines, i) ? pointer(lines[i]) : Ptr{DopAffineBC}(C_NULL), 3))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:dop_plan_create, libdop), Cint, (Ref{DopDesc}, Ref{Ptr{Cvoid}}), desc, h))
     p = Plan(h[], keep)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define DOP_ABI_VERSION 1
+#define DOP_ABI_VERSION 2
 
 /* error codes */
 #define DOP_OK 0
@@ -139,6 +139,11 @@ typedef struct dop_desc {
     int32_t transverse_view;
     const dop_operator* ops;          /* [n_ops], applied as sum_t */
     dop_affine_bc bc[3];              /* GHOST input: per axis */
+    /* GHOST input, optional per axis: one affine BC per line along that axis (MultiDimBC{dim}
+     * of an Array of atomic BCs, multi_dim_bc_operators.jl:93-95; line l of axis a is the
+     * column-major index over the other axes), or NULL for the single BC bc[a] (whose
+     * present flag must still be set; its taps are then unused). */
+    const dop_affine_bc* line_bc[3];
 } dop_desc;
 
 typedef struct dop_plan dop_plan;

@@ -719,6 +719,15 @@ int orc_ghost_apply(int nops, const orc_op* ops, int ndim, const long long* size
         const orc_bc* q = &bcs[ops[t].axis];
         if (q->nl < 0 || q->nr < 0 || q->nl > size[ops[t].axis] || q->nr > size[ops[t].axis])
             return ORC_EINVAL;
+        if (q->lines) {
+            const long long nl_lines = n / size[ops[t].axis];
+            for (long long l = 0; l < nl_lines; ++l) {
+                const orc_bc* e = &q->lines[l];
+                if (e->periodic || e->nl < 0 || e->nr < 0 || e->nl > size[ops[t].axis] ||
+                    e->nr > size[ops[t].axis])
+                    return ORC_EINVAL;
+            }
+        }
     }
     double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
     if (!tmp) return ORC_ENOMEM;
@@ -753,17 +762,18 @@ int orc_ghost_apply(int nops, const orc_op* ops, int ndim, const long long* size
                 /* Q*u_line: bc_operators.jl:188-191 (a_l . u[1:nl] + b_l), or the
                  * periodic wrap of :192 (u[end], u[1]) */
                 double gl = 0.0, gr = 0.0;
-                if (q->periodic == 1) {
+                const orc_bc* ql = q->lines ? &q->lines[ln] : q;
+                if (ql->periodic == 1) {
                     gl = xl[len];
                     gr = xl[1];
-                } else if (q->periodic == 2) {
+                } else if (ql->periodic == 2) {
                     gl = xl[1];
                     gr = xl[len];
                 } else {
-                    for (int k = 0; k < q->nl; ++k) gl += q->a_l[k] * xl[1 + k];
-                    for (int k = 0; k < q->nr; ++k) gr += q->a_r[k] * xl[1 + len - q->nr + k];
-                    gl = gl + q->b_l;
-                    gr = gr + q->b_r;
+                    for (int k = 0; k < ql->nl; ++k) gl += ql->a_l[k] * xl[1 + k];
+                    for (int k = 0; k < ql->nr; ++k) gr += ql->a_r[k] * xl[1 + len - ql->nr + k];
+                    gl = gl + ql->b_l;
+                    gr = gr + ql->b_r;
                 }
                 xl[0] = gl;
                 xl[len + 1] = gr;

@@ -76,6 +76,10 @@ typedef struct orc_bc {
     const double* a_r;
     double b_r;
     int periodic;
+    /* N-D: one affine BC per line along the axis (MultiDimBC{dim} of an Array of atomic
+     * BCs, multi_dim_bc_operators.jl:93-95; slice_rmul applies BC[I] to the line u[I, :, J],
+     * :34), lines in column-major order over the other axes; NULL: this BC on every line */
+    const struct orc_bc* lines;
 } orc_bc;
 
 /* fornberg.jl:7-62 */

@@ -33,10 +33,13 @@ class OrcOp(C.Structure):
 
 
 class OrcBC(C.Structure):
-    _fields_ = [
-        ("nl", C.c_int), ("nr", C.c_int), ("a_l", _dp), ("b_l", C.c_double),
-        ("a_r", _dp), ("b_r", C.c_double), ("periodic", C.c_int),
-    ]
+    pass
+
+
+OrcBC._fields_ = [
+    ("nl", C.c_int), ("nr", C.c_int), ("a_l", _dp), ("b_l", C.c_double),
+    ("a_r", _dp), ("b_r", C.c_double), ("periodic", C.c_int), ("lines", C.POINTER(OrcBC)),
+]
 
 
 _lib = None
@@ -181,15 +184,25 @@ class Op:
 class BC:
     """An affine (Robin-family) BC: ghost = a . u[1:k] + b on each side."""
 
-    def __init__(self, a_l, b_l, a_r, b_r, periodic=0):
+    def __init__(self, a_l, b_l, a_r, b_r, periodic=0, lines=None):
         self.a_l = np.ascontiguousarray(a_l, dtype=np.float64)
         self.a_r = np.ascontiguousarray(a_r, dtype=np.float64)
         self.b_l, self.b_r = float(b_l), float(b_r)
         self.periodic = int(periodic)   # 1: line wrap, 2: N-D periodic padding (first/last)
+        self.lines = lines              # N-D: one BC per line (column-major over the other axes)
 
     def struct(self) -> OrcBC:
-        return OrcBC(len(self.a_l), len(self.a_r), _ptr(self.a_l), self.b_l, _ptr(self.a_r), self.b_r,
-                     int(self.periodic))
+        s = OrcBC(len(self.a_l), len(self.a_r), _ptr(self.a_l), self.b_l, _ptr(self.a_r), self.b_r,
+                  int(self.periodic), None)
+        if self.lines is not None:
+            self._line_structs = (OrcBC * len(self.lines))(*[b.struct() for b in self.lines])
+            s.lines = C.cast(self._line_structs, C.POINTER(OrcBC))
+        return s
+
+    @staticmethod
+    def per_line(bcs):
+        """One BC per line (a list in column-major order over the other axes)."""
+        return BC([], 0.0, [], 0.0, lines=list(bcs))
 
     def pad(self, u):
         u = np.asarray(u, dtype=np.float64)

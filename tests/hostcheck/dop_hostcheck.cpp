@@ -39,6 +39,7 @@ dop::PlanDev bind(const dop::HostPlan& H)
         X.bJ0 = T.bJ0.empty() ? nullptr : T.bJ0.data();
         X.bn = T.bn.empty() ? nullptr : T.bn.data();
         X.bw = T.bw.empty() ? nullptr : T.bw.data();
+        X.lbc = T.lbc.empty() ? nullptr : T.lbc.data();
     }
     return P;
 }
