@@ -68,9 +68,10 @@ DOP_HD inline void row_entries(const PlanDev& P, const int64_t* idx, F& emit)
         for (int b = 0; b < P.ndim; ++b)
             if (b != a) base += idx[b] * P.ssrc[b];
         if (r >= X.rlo && r < X.rhi) {
-            const double* w = X.w64 + (X.uniform ? 0 : (int64_t)r * X.K);
+            const double* w = X.w64 + (X.uniform ? 0 : r);   // tap-major table
+            const int64_t ws = X.uniform ? 1 : X.n;
             const int64_t p0 = base + (int64_t)(r + X.dmin + (P.ghost ? 0 : 1)) * S;
-            for (int k = 0; k < X.K; ++k) emit(p0 + k * S, w[k]);
+            for (int k = 0; k < X.K; ++k) emit(p0 + k * S, w[k * ws]);
         } else {
             const int b = r < X.rlo ? r : X.rlo + (r - X.rhi);
             const int J0 = X.bJ0[b];

@@ -209,6 +209,41 @@ DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* s
     return s;
 }
 
+// The terms of axis X (row r, line origin `base`, stride S) added to acc: a regular row's K
+// taps in order, or a boundary row's FP64 value.  Shared by eval_point and the kernels that
+// stage a line elsewhere (same arithmetic, same order).
+template <typename T, int ACC>
+DOP_HD inline void axis_term(const PlanDev& P, const AxisDev& X, const T* src, int r, int64_t base,
+                             int64_t S, int64_t line, Acc<ACC>& acc)
+{
+    if (r >= X.rlo && r < X.rhi) {
+        // weights of row r: tap k at w[k * ws] (tap-major table; one row when uniform)
+        const auto* w = Acc<ACC>::table(X) + (X.uniform ? 0 : r);
+        const int64_t ws = X.uniform ? 1 : X.n;
+        const int64_t p0 = base + (int64_t)(r + X.dmin + (P.ghost ? 0 : 1)) * S;
+        // all taps' loads are issued before the first FMA (K <= kEvalUnroll covers every
+        // centered/upwind window of approximation order <= 8 without a loop)
+        if (X.K <= kEvalUnroll) {
+            typename Acc<ACC>::W wv[kEvalUnroll], xv[kEvalUnroll];
+#pragma unroll
+            for (int k = 0; k < kEvalUnroll; ++k) {
+                if (k < X.K) {
+                    wv[k] = w[k * ws];
+                    xv[k] = (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kEvalUnroll; ++k)
+                if (k < X.K) acc.add(wv[k], xv[k]);
+        } else {
+            for (int k = 0; k < X.K; ++k)
+                acc.add(w[k * ws], (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
+        }
+    } else {
+        acc.add_sum(boundary_row<T>(P, X, src, r, base, S, line));
+    }
+}
+
 // du value at multi-index idx (without the accumulate term)
 template <typename T, int ACC>
 DOP_HD inline double eval_point(const PlanDev& P, const T* src, const int64_t* idx)
@@ -217,35 +252,10 @@ DOP_HD inline double eval_point(const PlanDev& P, const T* src, const int64_t* i
     for (int a = 0; a < P.ndim; ++a) {
         const AxisDev& X = P.ax[a];
         if (!X.active) continue;
-        const int r = (int)idx[a];
-        const int64_t S = P.ssrc[a];
         int64_t base = P.tshift[a];
         for (int b = 0; b < P.ndim; ++b)
             if (b != a) base += idx[b] * P.ssrc[b];
-        if (r >= X.rlo && r < X.rhi) {
-            const auto* w = Acc<ACC>::table(X) + (X.uniform ? 0 : (int64_t)r * X.K);
-            const int64_t p0 = base + (int64_t)(r + X.dmin + (P.ghost ? 0 : 1)) * S;
-            // all taps' loads are issued before the first FMA (K <= kEvalUnroll covers
-            // every centered/upwind window of approximation order <= 8 without a loop)
-            if (X.K <= kEvalUnroll) {
-                typename Acc<ACC>::W wv[kEvalUnroll], xv[kEvalUnroll];
-#pragma unroll
-                for (int k = 0; k < kEvalUnroll; ++k) {
-                    if (k < X.K) {
-                        wv[k] = w[k];
-                        xv[k] = (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < kEvalUnroll; ++k)
-                    if (k < X.K) acc.add(wv[k], xv[k]);
-            } else {
-                for (int k = 0; k < X.K; ++k)
-                    acc.add(w[k], (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
-            }
-        } else {
-            acc.add_sum(boundary_row<T>(P, X, src, r, base, S, line_of(X, idx)));
-        }
+        axis_term<T, ACC>(P, X, src, (int)idx[a], base, P.ssrc[a], line_of(X, idx), acc);
     }
     return acc.value();
 }

@@ -105,7 +105,8 @@ void build_fast(const HostPlan& H, FastPlan* fp)
             const int src = X.uniform ? 0 : r;
             if (!X.uniform && (r < fp->nat_lo[a] || r >= fp->nat_hi[a])) continue;
             for (int k = 0; k < X.K; ++k)
-                fp->w64[a][(size_t)r * K + (X.dmin + k + half)] = T.w64[(size_t)src * X.K + k];
+                fp->w64[a][(size_t)r * K + (X.dmin + k + half)] =
+                    T.w64[X.uniform ? (size_t)k : (size_t)k * P.n[a] + src];
         }
         fp->w32[a].resize(fp->w64[a].size());
         for (size_t i = 0; i < fp->w64[a].size(); ++i) fp->w32[a][i] = (float)fp->w64[a][i];

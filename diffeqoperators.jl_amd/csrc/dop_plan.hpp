@@ -42,7 +42,7 @@ struct AxisDev {
     double al[kMaxBCTaps];
     double ar[kMaxBCTaps];
     double bl, br;
-    const double* w64;    // [uniform ? 1 : n][K]
+    const double* w64;    // uniform: [K]; else tap-major [K][n] (row r, tap k at k * n + r)
     const float* w32;     // same, rounded to f32
     const int32_t* bJ0;   // [nb] padded start
     const int32_t* bn;    // [nb] taps

@@ -288,10 +288,12 @@ bool build_axis(const std::vector<const dop_operator*>& ops, int sem, int n, Axi
         if (uni) {
             T.w64 = row0;
         } else {
+            // tap-major, w[k * n + r]: the threads of neighbouring rows read neighbouring
+            // words for every tap
             T.w64.assign((size_t)n * K, 0.0);
             for (int r = rlo; r < rhi; ++r) {
                 fill(r, rowr);
-                std::copy(rowr.begin(), rowr.end(), T.w64.begin() + (size_t)r * K);
+                for (int k = 0; k < K; ++k) T.w64[(size_t)k * n + r] = rowr[k];
             }
         }
         T.w32.resize(T.w64.size());
