@@ -162,27 +162,37 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    float best = 1e30f;
-    int best_v = 0;
+    // steady state, as an application sees it: back-to-back launches of one candidate between
+    // two events; the candidates take turns over three rounds (drift in clocks or
+    // temperature hits them alike) and each keeps its fastest round
     const char* vb = std::getenv("DOP_AUTOTUNE_VERBOSE");
     const bool verbose = vb && vb[0] == '1';
-    for (int v = 0; v < nv; ++v) {
-        float tsum = 0.0f;
-        bool ok = true;
-        for (int rep = 0; rep < 4 && ok; ++rep) {
+    constexpr int kReps = 8, kRounds = 3;
+    std::vector<float> per(nv, 1e30f);
+    std::vector<char> okv(nv, 1);
+    for (int v = 0; v < nv; ++v)   // warm-up (code objects, caches)
+        okv[v] = dop::launch_apply(plan->d, du, u, 0, v, 0, s) == hipSuccess;
+    for (int round = 0; round < kRounds; ++round) {
+        for (int v = 0; v < nv; ++v) {
+            if (!okv[v]) continue;
             (void)hipEventRecord(e0, s);
-            ok = dop::launch_apply(plan->d, du, u, 0, v, 0, s) == hipSuccess;
+            for (int rep = 0; rep < kReps && okv[v]; ++rep)
+                okv[v] = dop::launch_apply(plan->d, du, u, 0, v, 0, s) == hipSuccess;
             (void)hipEventRecord(e1, s);
             (void)hipEventSynchronize(e1);
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            if (rep > 0) tsum += ms;
+            per[v] = std::min(per[v], ms / kReps);
         }
+    }
+    float best = 1e30f;
+    int best_v = 0;
+    for (int v = 0; v < nv; ++v) {
         if (verbose)
-            fprintf(stderr, "dop_plan_autotune: variant %d %s %.4f ms\n", v, ok ? "ok" : "failed",
-                    tsum / 3.0f);
-        if (ok && tsum < best) {
-            best = tsum;
+            fprintf(stderr, "dop_plan_autotune: variant %d %s %.4f ms\n", v, okv[v] ? "ok" : "failed",
+                    per[v]);
+        if (okv[v] && per[v] < best) {
+            best = per[v];
             best_v = v;
         }
     }
