@@ -176,8 +176,10 @@ __device__ void tile2d(const BProb& B, const StageBufs<T>& S, int xg, int ch)
     const T* sidea = STAGE == -2 ? S.out : S.u;
     const T* sideb = S.acc;
     SC ea[R], eb[R];
+    // rows past the chunk (the prefetch of its last steps) reload the chunk's last row or
+    // last tap row, already in cache, instead of reading rows this wave never uses
     auto load_side = [&](int row, SC* a, SC* b) {
-        row = min(max(row, 0), n1 - 1);
+        row = min(max(row, 0), min(y1 - 1, n1 - 1));
         const int64_t p = (int64_t)row * s1 + min(max(x, (int64_t)0), n0 - 1);
         if constexpr (kSideA<STAGE>) *a = Store<T>::ld(sidea, p);
         if constexpr (kSideB<STAGE>) *b = Store<T>::ld(sideb, p);
@@ -187,7 +189,7 @@ __device__ void tile2d(const BProb& B, const StageBufs<T>& S, int xg, int ch)
     // stores nothing; its neighbours' x windows never reach it (the box is H from the edges)
     const int64_t xc = min(max(x, (int64_t)0), n0 - 1);
     auto load_row = [&](int row) -> W {
-        row = min(max(row, 0), n1 - 1);
+        row = min(max(row, 0), min(y1 + H - 1, n1 - 1));
         return (W)Store<T>::ld(S.X, (int64_t)row * s1 + xc);
     };
 #pragma unroll

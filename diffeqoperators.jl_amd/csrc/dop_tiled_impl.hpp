@@ -381,8 +381,11 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
 
     const T* ucol = u + xl;
     C q[R][VX];
+    // rows past the chunk's last tap row (the prefetch of its last steps) reload that row,
+    // already in cache, instead of reading rows this wave never uses
+    const int ylast = min(y1 + H - 1, n1 - 1);
     auto load_row = [&](int row, C* dst) {
-        row = min(max(row, 0), n1 - 1);
+        row = min(max(row, 0), ylast);
         VecIO<T, VX>::ld(ucol + (int64_t)row * s1, dst);
     };
 #pragma unroll
@@ -540,8 +543,11 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     const T* urow[VY];
 #pragma unroll
     for (int i = 0; i < VY; ++i) urow[i] = u + (int64_t)yc[i] * s1 + xl;
+    // planes past the chunk's last tap plane reload that plane (cached) instead of reading
+    // planes this wave never uses
+    const int zlast = min(z1 + H - 1, n2 - 1);
     auto load_plane = [&](int i, int zz, C* dst) {
-        zz = min(max(zz, 0), n2 - 1);
+        zz = min(max(zz, 0), zlast);
         VecIO<T, VX>::ld(urow[i] + (int64_t)zz * s2, dst);
     };
 
