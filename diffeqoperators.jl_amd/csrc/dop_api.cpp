@@ -335,6 +335,26 @@ int dop_batch_autotune(dop_batch* batch, void* stream, int* choice)
 
 void dop_batch_destroy(dop_batch* batch) { delete batch; }
 
+namespace {
+// One stage of a batch captured with its groups on ns forked streams (fresh events per stage:
+// each fork and join becomes its own graph edge).
+int capture_stage_forked(const dop_batch* b, int stage, hipStream_t cs, hipStream_t* ws, int ns,
+                         std::vector<hipEvent_t>* evs)
+{
+    const size_t first = evs->size();
+    for (int i = 0; i <= ns; ++i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return fail(DOP_EHIP, "rk4: event creation failed");
+        evs->push_back(e);
+    }
+    if (dop::launch_batch_items_forked(b->storage, b->d_probs, b->d_items, b->groups, stage, 0,
+                                       b->grid, cs, ws, evs->data() + first, ns) != hipSuccess)
+        return fail(DOP_EHIP, "rk4: stage launch during capture failed");
+    return DOP_OK;
+}
+}  // namespace
+
 int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, const double* dt,
                    void* const* u, void* stream, dop_rk4** out)
 {
@@ -372,17 +392,30 @@ int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, con
         if (at && at[0] == '0') r = batch_upload(&rk->b, rk->b.chunk);
         else r = batch_tune(&rk->b, 0, s, nullptr);
     }
-    // capture one step (four stage launches) into a graph
+    // capture one step (four stages; each stage's groups on forked streams) into a graph.
+    // DOP_RK4_STREAMS sets the number of streams (1: every group in order on one stream).
     hipStream_t cs = nullptr;
+    int ns = 4;
+    if (const char* e = std::getenv("DOP_RK4_STREAMS")) ns = std::max(1, std::min(8, std::atoi(e)));
+    std::vector<hipStream_t> ws(ns, nullptr);
+    std::vector<hipEvent_t> evs;
     if (r == DOP_OK && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
         r = fail(DOP_EHIP, "rk4: stream creation failed");
+    for (int i = 0; i < ns && r == DOP_OK; ++i)
+        if (hipStreamCreateWithFlags(&ws[i], hipStreamNonBlocking) != hipSuccess)
+            r = fail(DOP_EHIP, "rk4: stream creation failed");
     if (r == DOP_OK) {
         if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             r = fail(DOP_EHIP, "rk4: begin capture failed");
         } else {
-            for (int st = 0; st < 4; ++st)
-                if (batch_launch(&rk->b, st, 0, cs) != hipSuccess)
-                    r = fail(DOP_EHIP, "rk4: stage launch during capture failed");
+            for (int st = 0; st < 4 && r == DOP_OK; ++st) {
+                if (ns == 1) {
+                    if (batch_launch(&rk->b, st, 0, cs) != hipSuccess)
+                        r = fail(DOP_EHIP, "rk4: stage launch during capture failed");
+                } else {
+                    r = capture_stage_forked(&rk->b, st, cs, ws.data(), ns, &evs);
+                }
+            }
             if (hipStreamEndCapture(cs, &rk->graph) != hipSuccess && r == DOP_OK)
                 r = fail(DOP_EHIP, "rk4: end capture failed");
             if (r == DOP_OK &&
@@ -390,6 +423,9 @@ int dop_rk4_create(int n, const dop_plan* const* plans, const double* scale, con
                 r = fail(DOP_EHIP, "rk4: graph instantiation failed");
         }
     }
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    for (hipStream_t w : ws)
+        if (w) (void)hipStreamDestroy(w);
     if (cs) (void)hipStreamDestroy(cs);
     if (r != DOP_OK) {
         delete rk;

@@ -53,6 +53,14 @@ void batch_items(const std::vector<BProb>& probs, std::vector<BItem>* items,
                  std::vector<BGroup>* groups);
 
 // stage < 0: plain apply; 0..3: RK4 stage.  One launch per group, in order, on stream s.
+// one group's launch (the groups of a stage are independent: they may run concurrently)
+hipError_t launch_batch_group(int storage, const BGroup& g, const BProb* probs,
+                              const BItem* items, int stage, int accumulate, int grid,
+                              hipStream_t s);
+hipError_t launch_batch_items_forked(int storage, const BProb* probs, const BItem* items,
+                                     const std::vector<BGroup>& groups, int stage, int accumulate,
+                                     int grid, hipStream_t s, const hipStream_t* ws,
+                                     const hipEvent_t* evs, int ns);
 hipError_t launch_batch_items(int storage, const BProb* probs, const BItem* items,
                               const std::vector<BGroup>& groups, int stage, int accumulate,
                               int grid, hipStream_t s);
