@@ -433,13 +433,17 @@ int dop_tsit5_run(dop_tsit5* I, int64_t max_attempts, int check_every, void* str
     if (hipMemcpyAsync(&H, I->st, sizeof(H), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return tfail(DOP_EHIP, "tsit5: state read failed");
+    // replays between two reads of the state: the attempts the current step size still needs
+    // (plus one), at most `batch`, which doubles from 1 up to check_every -- the operator applies
+    // of a replay run even after the end time is reached, so launching far past it would be
+    // wasted work, and a step size that grows by up to 10x per step makes the first estimates
+    // far too large
+    int64_t batch = 1;
     while (left > 0 && !H.done) {
-        // replays between two reads of the state: the attempts the current step size still
-        // needs (plus one), at most check_every -- the operator applies of a replay run even
-        // after the end time is reached, so launching far past it would be wasted work
         const double need = H.dt > 0.0 ? std::ceil((H.t_end - H.t) / H.dt) + 1.0 : 1.0;
-        int64_t k = need < (double)check_every ? (int64_t)need : check_every;
+        int64_t k = need < (double)batch ? (int64_t)need : batch;
         if (k < 1) k = 1;
+        batch = batch * 2 < check_every ? batch * 2 : check_every;
         if (k > left) k = left;
         for (int64_t i = 0; i < k; ++i)
             if (hipGraphLaunch(I->exec, s) != hipSuccess) return tfail(DOP_EHIP, "tsit5: graph launch failed");

@@ -279,8 +279,9 @@ int dop_nonlinear_diffusion(const dop_plan* const* dq, const dop_plan* const* dp
  * steps are clamped to end exactly at t1.  dt0 <= 0 picks the starting step size by Hairer's
  * rule.  Every attempted step (six stages, six fused applies, the error reduction and the
  * controller) is one hipGraph replay with no host round trip; dop_tsit5_run replays up to
- * max_attempts of them, reading the device state after at most check_every replays (fewer
- * when the current step size reaches the end time sooner), then commits the last accepted
+ * max_attempts of them, reading the device state between batches of replays (1, 2, 4, ...
+ * up to check_every, and no more than the current step size needs to reach the end time,
+ * plus one), then commits the last accepted
  * step into u.  dop_tsit5_status: done = 1 at t1, -1 when the step size
  * collapsed.  dop_tsit5_set_end moves the end time (continuing from the current state). */
 typedef struct dop_tsit5 dop_tsit5;
