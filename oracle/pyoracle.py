@@ -6,6 +6,15 @@ imports this module.
 
 Arrays follow the reference's column-major order: a Julia array of size (n1, n2, n3)
 is a C-contiguous numpy array of shape (n3, n2, n1).
+
+Beyond the binding, restated here in numpy and pinned by the tests named with each:
+tsit5 (tests/test_oracle_tsit5.py: order conditions, matrix exponential), gradient /
+divergence / curl (tests/test_oracle_vector_calculus.py: the reference tests' analytic
+fields, curl of a gradient, a planted sign mistake), nonlinear_diffusion
+(tests/test_oracle_nonlinear.py: Fast_Diffusion.jl's closed form, p = 1, a planted swap).
+Parity unpinned: OrdinaryDiffEq's own Tsit5 step sequence (an absent dependency with no pinned
+version; only the published method is restated) and the vector-calculus transverse view 1:n on
+non-separable 2-D / 3-D fields (restated as read; the reference's tests use separable fields).
 """
 from __future__ import annotations
 
