@@ -119,8 +119,9 @@ void build_fast(const HostPlan& H, FastPlan* fp)
 }
 
 // Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap x
-// points per lane {widest 16-byte, twice that}; 3-D: chunk depth {64,32} x waves per block
-// {8,16} x XCD remap x points per lane {widest, half}.
+// points per lane {widest 16-byte, twice that}; 3-D: chunk depth {64,32} x block shape
+// (waves, rows per wave) {(8,1), (16,1), (8,2), (16,2), (8,3)} x XCD remap x points per lane
+// {widest, half}.  A block stages waves * rows y rows and computes all but the 2H halo rows.
 namespace {
 bool wide_ok(const FastPlan& fp) { return fp.VX > 1 && fp.args.n[0] % (2 * fp.VX) == 0; }
 }  // namespace
@@ -129,7 +130,7 @@ int fast_variant_count(const FastPlan& fp)
 {
     if (fp.kind == 1) return 1;
     if (fp.kind == 2) return wide_ok(fp) ? 32 : 16;
-    if (fp.kind == 3) return fp.VX > 1 ? 32 : 16;
+    if (fp.kind == 3) return fp.VX > 1 ? 40 : 20;
     return 1;
 }
 
@@ -149,10 +150,12 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         static const int chunks[2] = {64, 32};
         A.chunk = chunks[(v >> 1) & 1];
         A.PF = 1;
-        A.W = ((v >> 2) & 1) ? 16 : 8;
-        A.VY = ((v >> 3) & 1) ? 2 : 1;
+        static const int shapes[5][2] = {{8, 1}, {16, 1}, {8, 2}, {16, 2}, {8, 3}};
+        const int sh = (v >> 2) % 5;
+        A.W = shapes[sh][0];
+        A.VY = shapes[sh][1];
         if (A.VY == 1 && 2 * (A.K / 2) >= 8) A.W = 16;   // a block must compute some rows
-        A.VX = ((v >> 4) & 1) ? fp.VX / 2 : fp.VX;
+        A.VX = ((v >> 2) / 5) ? fp.VX / 2 : fp.VX;
     }
     {
         // segments of 64*VX points overlapping by 2*PADL (the x halo, whole lanes)

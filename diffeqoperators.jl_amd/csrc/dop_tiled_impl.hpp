@@ -670,26 +670,27 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
     hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, kPF3D<T, K>, W, VY>), grid, dim3(64 * W), 0, s, \
                        A, p.dev, fp.shell, nshell, (T*)du, (const T*)u, accumulate)
             constexpr bool w8 = 2 * (K / 2) < 8;   // 8-wave blocks of one row compute 8 - 2H rows
-            if (A.W != 16 && !w8 && A.VY == 1) return hipErrorInvalidValue;
-            if (A.VY == 2) {
-                if (uni) {
-                    if (A.W == 8) DOP_L3(true, 8, 2); else DOP_L3(true, 16, 2);
-                } else {
-                    if (A.W == 8) DOP_L3(false, 8, 2); else DOP_L3(false, 16, 2);
-                }
-            } else if (uni) {
-                if constexpr (w8) {
-                    if (A.W == 8) DOP_L3(true, 8, 1); else DOP_L3(true, 16, 1);
-                } else {
-                    DOP_L3(true, 16, 1);
-                }
+#define DOP_L3U(W, VY)                                                                       \
+    do {                                                                                     \
+        if (uni) DOP_L3(true, W, VY); else DOP_L3(false, W, VY);                             \
+    } while (0)
+            // (waves per block, rows per wave): the shapes fast_variant produces
+            const int shape = A.W * 8 + A.VY;
+            if (shape == 8 * 8 + 1) {
+                if constexpr (w8) DOP_L3U(8, 1);
+                else return hipErrorInvalidValue;
+            } else if (shape == 16 * 8 + 1) {
+                DOP_L3U(16, 1);
+            } else if (shape == 8 * 8 + 2) {
+                DOP_L3U(8, 2);
+            } else if (shape == 16 * 8 + 2) {
+                DOP_L3U(16, 2);
+            } else if (shape == 8 * 8 + 3) {
+                DOP_L3U(8, 3);
             } else {
-                if constexpr (w8) {
-                    if (A.W == 8) DOP_L3(false, 8, 1); else DOP_L3(false, 16, 1);
-                } else {
-                    DOP_L3(false, 16, 1);
-                }
+                return hipErrorInvalidValue;
             }
+#undef DOP_L3U
 #undef DOP_L3
         } else {
             return hipErrorInvalidValue;
