@@ -353,8 +353,9 @@ def main():
                     help="also run a plain 256 MiB device copy (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this build has no "
-              "multi-GPU path; see DESIGN.md for how the operator would shard.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): there is no multi-GPU "
+              "path (the operator would shard as slabs along the slowest axis, one halo "
+              "exchange per apply).", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
