@@ -97,6 +97,29 @@ def bench_laplacian2d(steps, warmup):
         alg_bytes=alg_bytes, variant=variant)
 
 
+def bench_laplacian2d_f32(steps, warmup):
+    """The same 2-D workload with FP32 storage and FP32 accumulation, and its normwise error
+    against the FP64 apply of the same (rounded) field."""
+    import torch
+    import diffeqoperators_amd as D
+    n = 4096
+    A, u64, _, _ = laplacian2d(n)
+    u = u64.float()
+    plan = D.plans_for(A, u)[0]
+    variant = plan.autotune()
+    du = torch.empty_like(u)
+    wall, ev = time_applies(lambda: plan.apply(du, u), steps, warmup)
+    ref = torch.empty_like(u64)
+    D.plans_for(A, u64)[0].apply(ref, u.double())
+    err = ((du.double() - ref).abs().max() / ref.abs().max()).item()
+    pts = n * n
+    kern_s = ev / steps
+    achieved = 2 * 4 * pts / kern_s / 1e9
+    return dict(gpts_s=round(pts * steps / wall / 1e9, 2), kernel_ms=round(kern_s * 1e3, 4),
+                achieved_GBs=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4),
+                max_rel_err_vs_f64=float(f"{err:.3e}"), tile_variant=variant)
+
+
 def cpu_baseline_laplacian2d(seconds):
     """The oracle (FP64 C restatement of the reference's per-line loops) on the same
     workload, one thread, on a bounded sample: whole applies until `seconds` elapse."""
@@ -415,7 +438,9 @@ def main():
             "variants": r3,
         }
     if not args.no_also and args.config == "laplacian2d":
-        line["also"] = {"nonuniform3d_512": bench_nonuniform3d(max(20, args.steps // 4), 3)}
+        line["also"] = {"laplacian2d_4096_f32": bench_laplacian2d_f32(args.steps, args.warmup)}
+        torch.cuda.empty_cache()
+        line["also"]["nonuniform3d_512"] = bench_nonuniform3d(max(20, args.steps // 4), 3)
         torch.cuda.empty_cache()
         line["also"]["batched_rk4_1024"] = bench_batched(max(10, args.steps // 10))
         torch.cuda.empty_cache()
