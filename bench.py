@@ -120,6 +120,67 @@ def bench_laplacian2d_f32(steps, warmup):
                 max_rel_err_vs_f64=float(f"{err:.3e}"), tile_variant=variant)
 
 
+def bench_widened(steps=50, warmup=5):
+    """Throughput of the callers beside the application path: Gradient / Divergence / Curl of
+    a 256^3 FP32 field (padded input), nonlinear_diffusion! over 2^24 FP64 knots, and the 2-D
+    workload with one Robin BC per line along x.  Algorithmic bytes: every input array read once
+    and every output written once."""
+    import numpy as np
+    import torch
+    import diffeqoperators_amd as D
+    out = {}
+
+    def rec(name, fn, pts, alg_bytes):
+        wall, ev = time_applies(fn, steps, warmup)
+        t = ev / steps
+        ach = alg_bytes / t / 1e9
+        out[name] = dict(gpts_s=round(pts / t / 1e9, 2), kernel_ms=round(t * 1e3, 4),
+                         achieved_GBs=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4))
+
+    n = 256
+    h = 1.0 / (n + 1)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    M = torch.rand((n + 2,) * 3, device="cuda", generator=g) * 2 - 1
+    V = torch.rand((3,) + (n + 2,) * 3, device="cuda", generator=g) * 2 - 1
+    G = D.Gradient(4, (h, h, h), (n, n, n))
+    Dv = D.Divergence(4, (h, h, h), (n, n, n))
+    Cu = D.Curl(4, (h, h, h), (n, n, n))
+    rec("gradient_256_f32", lambda: G * M, n ** 3, 4 * ((n + 2) ** 3 + 3 * n ** 3))
+    rec("divergence_256_f32", lambda: Dv * V, n ** 3, 4 * (3 * (n + 2) ** 3 + n ** 3))
+    rec("curl_256_f32", lambda: Cu * V, n ** 3, 4 * (3 * (n + 2) ** 3 + 3 * n ** 3))
+    del M, V
+    torch.cuda.empty_cache()
+
+    m = 1 << 24
+    x = torch.linspace(-5.0, 5.0, m + 2, dtype=torch.float64, device="cuda")
+    q = 1.0 / torch.sqrt(x * x + 1.0)
+    pk = q ** -2
+    du = torch.empty(m, dtype=torch.float64, device="cuda")
+    dxn = 10.0 / (m + 1)
+    rec("nonlinear_diffusion_2e24_f64",
+        lambda: D.nonlinear_diffusion_(du, 1, 1, 2, pk, q, dxn, m), m, 8 * (2 * (m + 2) + m))
+    del x, q, pk, du
+    torch.cuda.empty_cache()
+
+    nn = 4096
+    hh = 1.0 / (nn + 1)
+    rng = np.random.default_rng(7)
+    lines = np.empty((nn,), dtype=object)
+    for i in range(nn):
+        lines[i] = D.RobinBC((1.0, float(rng.uniform(0.2, 1.0)), float(rng.standard_normal())),
+                             (1.0, float(rng.uniform(0.2, 1.0)), float(rng.standard_normal())),
+                             hh, 1)
+    Q = D.compose(D.MultiDimBC[1](lines),
+                  D.MultiDimBC[2](D.RobinBC((1.0, 0.5, 0.0), (1.0, 0.5, 0.0), hh, 1), (nn, nn)))
+    A = (D.CenteredDifference[1](2, 4, hh, nn) + D.CenteredDifference[2](2, 4, hh, nn)) * Q
+    u = torch.from_numpy(rng.uniform(-1.0, 1.0, (nn, nn))).to("cuda")
+    plan = D.plans_for(A, u)[0]
+    plan.autotune()
+    dv = torch.empty_like(u)
+    rec("laplacian2d_4096_f64_per_line_robin", lambda: plan.apply(dv, u), nn * nn, 2 * 8 * nn * nn)
+    return out
+
+
 def cpu_baseline_laplacian2d(seconds):
     """The oracle (FP64 C restatement of the reference's per-line loops) on the same
     workload, one thread, on a bounded sample: whole applies until `seconds` elapse."""
@@ -441,6 +502,11 @@ def main():
         line["also"] = {"laplacian2d_4096_f32": bench_laplacian2d_f32(args.steps, args.warmup)}
         torch.cuda.empty_cache()
         line["also"]["nonuniform3d_512"] = bench_nonuniform3d(max(20, args.steps // 4), 3)
+        torch.cuda.empty_cache()
+        try:
+            line["also"]["widened"] = bench_widened()
+        except Exception as e:   # reported, never fatal to the line
+            line["also"]["widened"] = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()
         line["also"]["batched_rk4_1024"] = bench_batched(max(10, args.steps // 10))
         torch.cuda.empty_cache()
