@@ -89,9 +89,10 @@ int dop_plan_create(const dop_desc* desc, dop_plan** out)
     (void)hipGetDevice(&p->device);
     int r = DOP_OK;
     // DOP_FORCE_GENERIC=1 (read when a plan is created) keeps every point on the generic
-    // kernel: the reference for checking the tiled kernels bit for bit.
+    // kernel: the reference for checking the tiled and single-axis kernels bit for bit.
     const char* fg = std::getenv("DOP_FORCE_GENERIC");
-    if (!(fg && fg[0] == '1')) dop::build_fast(H, &p->d.fast);
+    p->d.generic_only = (fg && fg[0] == '1') ? 1 : 0;
+    if (!p->d.generic_only) dop::build_fast(H, &p->d.fast);
     // DOP_TILE_VARIANT=v pins the tile-shape candidate (otherwise 0 until dop_plan_autotune)
     if (const char* tv = std::getenv("DOP_TILE_VARIANT")) {
         const int v = std::atoi(tv);

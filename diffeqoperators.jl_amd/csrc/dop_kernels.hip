@@ -35,10 +35,105 @@ __global__ __launch_bounds__(kBlock) void k_apply_generic(const PlanDev* __restr
     }
 }
 
+// one active axis (1-D plans, the padded plans of the vector calculus and nonlinear_diffusion!):
+// eval_point's arithmetic on that axis -- a regular row's K taps in order, a boundary row's
+// FP64 value -- with the window width a template constant, 32-bit point indices and no loop
+// over axes (bitwise equal to k_apply_generic)
+template <typename T, int ACC, int K>
+__global__ __launch_bounds__(kBlock) void k_apply_line(const PlanDev* __restrict__ Pp,
+                                                      T* __restrict__ du,
+                                                      const T* __restrict__ src, int accumulate,
+                                                      int a)
+{
+    using W = typename Acc<ACC>::W;
+    const PlanDev& P = *Pp;
+    const AxisDev& X = P.ax[a];
+    const uint32_t total = (uint32_t)P.total;
+    const uint32_t n0 = (uint32_t)P.n[0], n1 = (uint32_t)P.n[1];
+    const int64_t S = P.ssrc[a];
+    const W* wt = Acc<ACC>::table(X);
+    const int64_t ws = X.uniform ? 1 : X.n;
+    const int jd = X.dmin + (P.ghost ? 0 : 1);
+    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < total; p += gridDim.x * kBlock) {
+        const uint32_t q = p / n0;
+        const uint32_t q2 = q / n1;
+        int64_t idx[3];
+        idx[0] = p - q * n0;
+        idx[1] = q - q2 * n1;
+        idx[2] = q2;
+        int64_t base = P.tshift[a];
+#pragma unroll
+        for (int b = 0; b < kMaxDim; ++b)
+            if (b != a && b < P.ndim) base += idx[b] * P.ssrc[b];
+        const int r = (int)idx[a];
+        Acc<ACC> acc;
+        if (r >= X.rlo && r < X.rhi) {
+            const W* w = wt + (X.uniform ? 0 : r);
+            const int64_t p0 = base + (int64_t)(r + jd) * S;
+            W wv[K], xv[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                wv[k] = w[k * ws];
+                xv[k] = (W)Store<T>::ld(src, p0 + k * S);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc.add(wv[k], xv[k]);
+        } else {
+            acc.add_sum(boundary_row<T>(P, X, src, r, base, S, line_of(X, idx)));
+        }
+        double v = acc.value();
+        if (accumulate) v += (double)Store<T>::ld(du, p);
+        Store<T>::st(du, p, v);
+    }
+}
+
+template <typename T, int ACC>
+static hipError_t launch_line(const PlanDev& host, const PlanDev* dev, void* du, const void* u,
+                              int accumulate, int a, hipStream_t s)
+{
+    int64_t blocks = (host.total + kBlock - 1) / kBlock;
+    if (blocks > (1 << 22)) blocks = 1 << 22;
+    if (blocks < 1) blocks = 1;
+#define DOP_LINE(KK)                                                                         \
+    hipLaunchKernelGGL((k_apply_line<T, ACC, KK>), dim3((unsigned)blocks), dim3(kBlock), 0, s, \
+                       dev, (T*)du, (const T*)u, accumulate, a)
+    switch (host.ax[a].K) {
+    case 2: DOP_LINE(2); break;
+    case 3: DOP_LINE(3); break;
+    case 4: DOP_LINE(4); break;
+    case 5: DOP_LINE(5); break;
+    case 6: DOP_LINE(6); break;
+    case 7: DOP_LINE(7); break;
+    case 8: DOP_LINE(8); break;
+    case 9: DOP_LINE(9); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef DOP_LINE
+    return hipGetLastError();
+}
+
+// the single active axis of a plan the line kernel can take, or -1
+static int line_axis(const PlanDev& h)
+{
+    if (h.total >= ((int64_t)1 << 31)) return -1;
+    int a = -1;
+    for (int b = 0; b < h.ndim; ++b) {
+        if (!h.ax[b].active) continue;
+        if (a >= 0) return -1;
+        a = b;
+    }
+    if (a < 0 || h.ax[a].K < 2 || h.ax[a].K > 9 || h.ax[a].rhi <= h.ax[a].rlo) return -1;
+    return a;
+}
+
 template <typename T, int ACC>
 static hipError_t launch_generic(const PlanDev& host, const PlanDev* dev, void* du,
-                                 const void* u, int accumulate, hipStream_t s)
+                                 const void* u, int accumulate, hipStream_t s, int line = 1)
 {
+    if (line) {
+        const int a = line_axis(host);
+        if (a >= 0) return launch_line<T, ACC>(host, dev, du, u, accumulate, a, s);
+    }
     int64_t blocks = (host.total + kBlock - 1) / kBlock;
     if (blocks > (1 << 22)) blocks = 1 << 22;
     if (blocks < 1) blocks = 1;
@@ -53,25 +148,25 @@ hipError_t launch_tiled(const DevicePlan& p, void* du, const void* u, int accumu
                         hipStream_t s);
 
 static hipError_t launch_generic_any(const PlanDev& host, const PlanDev* dev, void* du,
-                                     const void* u, int accumulate, hipStream_t s)
+                                     const void* u, int accumulate, hipStream_t s, int line)
 {
     switch (host.storage) {
     case kF64:
-        return launch_generic<double, kAccF64>(host, dev, du, u, accumulate, s);
+        return launch_generic<double, kAccF64>(host, dev, du, u, accumulate, s, line);
     case kF32:
-        if (host.accum == kAccF64) return launch_generic<float, kAccF64>(host, dev, du, u, accumulate, s);
+        if (host.accum == kAccF64) return launch_generic<float, kAccF64>(host, dev, du, u, accumulate, s, line);
         if (host.accum == kAccF32Kahan)
-            return launch_generic<float, kAccF32Kahan>(host, dev, du, u, accumulate, s);
+            return launch_generic<float, kAccF32Kahan>(host, dev, du, u, accumulate, s, line);
         if (host.accum == kAccF32Wide)
-            return launch_generic<float, kAccF32Wide>(host, dev, du, u, accumulate, s);
-        return launch_generic<float, kAccF32>(host, dev, du, u, accumulate, s);
+            return launch_generic<float, kAccF32Wide>(host, dev, du, u, accumulate, s, line);
+        return launch_generic<float, kAccF32>(host, dev, du, u, accumulate, s, line);
     case kBF16:
-        if (host.accum == kAccF64) return launch_generic<bf16, kAccF64>(host, dev, du, u, accumulate, s);
+        if (host.accum == kAccF64) return launch_generic<bf16, kAccF64>(host, dev, du, u, accumulate, s, line);
         if (host.accum == kAccF32Kahan)
-            return launch_generic<bf16, kAccF32Kahan>(host, dev, du, u, accumulate, s);
+            return launch_generic<bf16, kAccF32Kahan>(host, dev, du, u, accumulate, s, line);
         if (host.accum == kAccF32Wide)
-            return launch_generic<bf16, kAccF32Wide>(host, dev, du, u, accumulate, s);
-        return launch_generic<bf16, kAccF32>(host, dev, du, u, accumulate, s);
+            return launch_generic<bf16, kAccF32Wide>(host, dev, du, u, accumulate, s, line);
+        return launch_generic<bf16, kAccF32>(host, dev, du, u, accumulate, s, line);
     }
     return hipErrorInvalidValue;
 }
@@ -83,7 +178,8 @@ hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumu
     const bool aligned = ((uintptr_t)du % 16 == 0) && ((uintptr_t)u % 16 == 0);
     if (p.fast.kind >= 2 && !p.fast.kahan && !force_generic && aligned)
         return launch_tiled(p, du, u, accumulate, variant < 0 ? p.variant : variant, s);
-    return launch_generic_any(p.host, p.dev, du, u, accumulate, s);
+    return launch_generic_any(p.host, p.dev, du, u, accumulate, s,
+                              !force_generic && !p.generic_only);
 }
 
 }  // namespace dop

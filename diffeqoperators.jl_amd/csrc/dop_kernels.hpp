@@ -14,13 +14,15 @@ struct DevicePlan {
     const PlanDev* dev = nullptr;
     FastPlan fast;
     int variant = 0;   // tile-shape candidate chosen by the autotuner
+    int generic_only = 0;   // every point on k_apply_generic (no tiles, no single-axis kernel)
 };
 
 constexpr int kBlock = 256;        // threads per block
 constexpr int kChunkPts = 1024;    // points per chunk of a persistent launch
 
-// Single-plan apply: the tiled interior + shell when the plan has one, else generic.
-// variant < 0 uses the plan's tuned variant; force_generic = 1 runs the generic kernel.
+// Single-plan apply: the tiled interior + shell when the plan has one, else the single-axis
+// kernel when one axis carries operators, else generic.  variant < 0 uses the plan's tuned
+// variant; force_generic = 1 (or the plan's generic_only) runs the generic kernel.
 hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumulate,
                         int variant, int force_generic, hipStream_t s);
 
