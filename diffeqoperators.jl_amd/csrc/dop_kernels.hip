@@ -39,6 +39,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_generic(const PlanDev* __restr
 // eval_point's arithmetic on that axis -- a regular row's K taps in order, a boundary row's
 // FP64 value -- with the window width a template constant, 32-bit point indices and no loop
 // over axes (bitwise equal to k_apply_generic)
+constexpr int64_t kLineBlocks = 8192;
+
 template <typename T, int ACC, int K>
 __global__ __launch_bounds__(kBlock) void k_apply_line(const PlanDev* __restrict__ Pp,
                                                       T* __restrict__ du,
@@ -54,20 +56,31 @@ __global__ __launch_bounds__(kBlock) void k_apply_line(const PlanDev* __restrict
     const W* wt = Acc<ACC>::table(X);
     const int64_t ws = X.uniform ? 1 : X.n;
     const int jd = X.dmin + (P.ghost ? 0 : 1);
-    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < total; p += gridDim.x * kBlock) {
+    // a persistent grid: block bx walks its own contiguous span of points (long-lived waves; a
+    // wave per point batch would spend its life being launched), and blocks are dispatched to
+    // the 8 XCDs in turn, so XCD x takes the x-th contiguous eighth of the spans: neighbouring
+    // rows and planes (the taps of strided axes) meet in the same L2
+    const uint32_t nb = gridDim.x, per = nb >> 3;
+    const uint32_t bx = blockIdx.x < per * 8 ? (blockIdx.x & 7) * per + (blockIdx.x >> 3) : blockIdx.x;
+    const uint32_t span = ((total + nb - 1) / nb + kBlock - 1) / kBlock * kBlock;
+    const uint32_t pend = min(total, (uint32_t)min((uint64_t)(bx + 1) * span, (uint64_t)total));
+    // the plan's fields the loop needs, read once (the point loop then issues no scalar loads)
+    const int64_t tsh = P.tshift[a];
+    int64_t sst[kMaxDim];
+#pragma unroll
+    for (int b = 0; b < kMaxDim; ++b) sst[b] = (b != a && b < P.ndim) ? P.ssrc[b] : 0;
+    const int rlo = X.rlo, rhi = X.rhi;
+    for (uint32_t p = bx * span + threadIdx.x; p < pend; p += kBlock) {
         const uint32_t q = p / n0;
         const uint32_t q2 = q / n1;
         int64_t idx[3];
         idx[0] = p - q * n0;
         idx[1] = q - q2 * n1;
         idx[2] = q2;
-        int64_t base = P.tshift[a];
-#pragma unroll
-        for (int b = 0; b < kMaxDim; ++b)
-            if (b != a && b < P.ndim) base += idx[b] * P.ssrc[b];
+        const int64_t base = tsh + idx[0] * sst[0] + idx[1] * sst[1] + idx[2] * sst[2];
         const int r = (int)idx[a];
         Acc<ACC> acc;
-        if (r >= X.rlo && r < X.rhi) {
+        if (r >= rlo && r < rhi) {
             const W* w = wt + (X.uniform ? 0 : r);
             const int64_t p0 = base + (int64_t)(r + jd) * S;
             W wv[K], xv[K];
@@ -91,8 +104,9 @@ template <typename T, int ACC>
 static hipError_t launch_line(const PlanDev& host, const PlanDev* dev, void* du, const void* u,
                               int accumulate, int a, hipStream_t s)
 {
+    // enough resident blocks for every CU several times over, each with a long span
     int64_t blocks = (host.total + kBlock - 1) / kBlock;
-    if (blocks > (1 << 22)) blocks = 1 << 22;
+    if (blocks > kLineBlocks) blocks = kLineBlocks;
     if (blocks < 1) blocks = 1;
 #define DOP_LINE(KK)                                                                         \
     hipLaunchKernelGGL((k_apply_line<T, ACC, KK>), dim3((unsigned)blocks), dim3(kBlock), 0, s, \
