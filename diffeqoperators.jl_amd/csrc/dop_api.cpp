@@ -168,11 +168,24 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
     // temperature hits them alike) and each keeps its fastest round
     const char* vb = std::getenv("DOP_AUTOTUNE_VERBOSE");
     const bool verbose = vb && vb[0] == '1';
-    constexpr int kReps = 8, kRounds = 3;
+    constexpr int kRounds = 3;
     std::vector<float> per(nv, 1e30f);
     std::vector<char> okv(nv, 1);
     for (int v = 0; v < nv; ++v)   // warm-up (code objects, caches)
         okv[v] = dop::launch_apply(plan->d, du, u, 0, v, 0, s) == hipSuccess;
+    // launches per measurement: enough for about 1 ms of back-to-back work (8 to 64), so that
+    // short kernels are timed in their steady state rather than by their first launches
+    int kReps = 8;
+    {
+        (void)hipEventRecord(e0, s);
+        for (int rep = 0; rep < 4; ++rep) (void)dop::launch_apply(plan->d, du, u, 0, 0, 0, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const float t = ms / 4;
+        if (t > 0.0f) kReps = std::max(8, std::min(64, (int)(1.0f / t) + 1));
+    }
     for (int round = 0; round < kRounds; ++round) {
         for (int v = 0; v < nv; ++v) {
             if (!okv[v]) continue;
