@@ -235,6 +235,42 @@ def nonuniform3d(n, dtype):
     return A, u
 
 
+def bench_slab(steps=50, warmup=5, nz=64, halo=3):
+    """One rank's share of the 512^3 FP32 workload sharded in z over 8 GPUs, at its real shape:
+    512 x 512 x (64 + 2*3) planes (the 3 halo planes per side are part of the array).  Applied
+    as a 70-plane problem, whose 3 edge planes per side are evaluated as boundary rows where a
+    rank would read halo planes: an upper bound on the per-rank compute."""
+    import numpy as np
+    import torch
+    import diffeqoperators_amd as D
+    n = 512
+    h = 1.0 / (n + 1)
+    sizes = (n, n, nz + 2 * halo)
+    ops = []
+    for k in (1, 2, 3):
+        m = sizes[k - 1]
+        dx = h * (1 + 0.5 * np.random.default_rng(10 + k).random(m + 1))
+        v = np.random.default_rng(20 + k).standard_normal(m)
+        ops += [D.CenteredDifference[k](2, 6, dx, m), D.UpwindDifference[k](1, 3, dx, m, -v)]
+    A = ops[0]
+    for L in ops[1:]:
+        A = A + L
+    A = A * D.compose(*D.Dirichlet0BC(float, sizes))
+    g = torch.Generator(device="cuda").manual_seed(3)
+    u = torch.rand(tuple(reversed(sizes)), device="cuda", generator=g) * 2 - 1
+    plan = D.plans_for(A, u)[0]
+    variant = plan.autotune()
+    du = torch.empty_like(u)
+    wall, ev = time_applies(lambda: plan.apply(du, u), steps, warmup)
+    t = ev / steps
+    pts = n * n * nz
+    ach = 2 * 4 * u.numel() / t / 1e9
+    return dict(shape=list(reversed(sizes)), owned_points=pts, kernel_us=round(t * 1e6, 1),
+                owned_gpts_s=round(pts / t / 1e9, 2), achieved_GBs=round(ach, 1),
+                frac=round(ach / HBM_PEAK_GBS, 4), tile_variant=variant,
+                halo_face_MB=round(n * n * halo * 4 / 1e6, 2))
+
+
 def bench_nonuniform3d(steps, warmup, n=512):
     import torch
     import diffeqoperators_amd as D
@@ -502,6 +538,8 @@ def main():
         line["also"] = {"laplacian2d_4096_f32": bench_laplacian2d_f32(args.steps, args.warmup)}
         torch.cuda.empty_cache()
         line["also"]["nonuniform3d_512"] = bench_nonuniform3d(max(20, args.steps // 4), 3)
+        torch.cuda.empty_cache()
+        line["also"]["slab_512x512x70_f32"] = bench_slab()
         torch.cuda.empty_cache()
         try:
             line["also"]["widened"] = bench_widened()
