@@ -1,13 +1,15 @@
 #!/usr/bin/env python3
 """Throughput of the fused operator application on one MI355X.
 
-Default workload (`--config laplacian2d`): the 2-D uniform Laplacian
-CenteredDifference{1}(2,4,h,4096) + CenteredDifference{2}(2,4,h,4096) composed with
-RobinBC(α=1, β=0.5, γ=0) on every edge, u a 4096×4096 FP64 field seeded uniform in [-1,1]:
-one step = one Δ*Q*u (one kernel launch).  Prints ONE JSON line.
+Default workload (`--config nonuniform3d`): the 3-D non-uniform 512^3 field, per axis k
+CenteredDifference{k}(2,6,dx_k,512) + UpwindDifference{k}(1,3,dx_k,512,-v_k) summed and composed
+with Dirichlet0BC on every face, FP32 storage and accumulation: one step = one (sum L)*Q*u.
+The FP64 and BF16 storage variants are timed beside it, each with its error against the FP64
+oracle on the same quantized input, and the oracle is timed on the same workload (the CPU
+baseline).  Prints ONE JSON line.
 
-Other workloads are reported in the same line under "also" (`--also`), or selected with
-`--config nonuniform3d|batched|served`.
+Other workloads are reported in the same line under "also" (`--no-also` skips them), or
+selected with `--config laplacian2d|batched|served`.
 """
 import argparse
 import json
@@ -271,25 +273,109 @@ def bench_slab(steps=50, warmup=5, nz=64, halo=3):
                 halo_face_MB=round(n * n * halo * 4 / 1e6, 2))
 
 
-def bench_nonuniform3d(steps, warmup, n=512):
+def nonuniform3d_oracle_ops(n):
+    """The oracle's operators of the 512^3 workload (the same seeded dx and wind)."""
+    import numpy as np
+    import pyoracle as po
+    h = 1.0 / (n + 1)
+    ops = []
+    for k in (1, 2, 3):
+        dx = h * (1 + 0.5 * np.random.default_rng(10 + k).random(n + 1))
+        v = np.random.default_rng(20 + k).standard_normal(n)
+        ops += [po.Op(0, 2, 6, dx, n, axis=k - 1), po.Op(1, 1, 3, dx, n, axis=k - 1, coeff=-v)]
+    return ops, [po.dirichlet0()] * 3
+
+
+def host_threads():
+    """(threads used, host cores): the cores this process may run on, capped by
+    OMP_NUM_THREADS when the environment sets it (the share of the host given to one GPU)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    use = min(avail, int(env)) if env and env.isdigit() and int(env) > 0 else avail
+    return max(1, use), os.cpu_count() or avail
+
+
+def rel_errors(du, ref):
+    """normwise |du - ref|_inf / |ref|_inf and pointwise max |du - ref| / max(|ref|,
+    1e-3 |ref|_inf)"""
+    import numpy as np
+    d = np.abs(du - ref)
+    nr = float(np.abs(ref).max())
+    return float(d.max() / nr), float((d / np.maximum(np.abs(ref), 1e-3 * nr)).max())
+
+
+def bench_nonuniform3d(steps, warmup, n=512, errors=True, cpu_seconds=12.0,
+                       types=("f32", "f64", "bf16")):
+    """The 512^3 non-uniform workload in FP64 / FP32 / BF16 storage (FP32 accumulation for the
+    narrow types), each with its error against the FP64 oracle on the same quantized input, and
+    the oracle itself timed on the same workload (the CPU baseline)."""
+    import numpy as np
     import torch
     import diffeqoperators_amd as D
-    out = {}
-    for name, dt, sb in (("f64", torch.float64, 8), ("f32", torch.float32, 4), ("bf16", torch.bfloat16, 2)):
+    out, dus, us = {}, {}, {}
+    pts = n ** 3
+    for name, dt, sb in (("f32", torch.float32, 4), ("f64", torch.float64, 8),
+                         ("bf16", torch.bfloat16, 2)):
+        if name not in types:
+            continue
         A, u = nonuniform3d(n, dt)
         plan = D.plans_for(A, u)[0]
-        variant = plan.autotune()
+        # a tile variant pinned by DOP_TILE_VARIANT (profiling runs) is kept
+        variant = plan.variant[0] if os.environ.get("DOP_TILE_VARIANT") else plan.autotune()
         du = torch.empty_like(u)
         wall, ev = time_applies(lambda: plan.apply(du, u), steps, warmup)
-        pts = n ** 3
         kern = ev / steps
         ach = 2 * sb * pts / kern / 1e9
-        out[name] = dict(gpts_s=round(pts / kern / 1e9, 2), kernel_ms=round(kern * 1e3, 4),
-                         achieved_GBs=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4),
-                         tile_variant=variant)
+        out[name] = dict(gpts_s=round(pts * steps / wall / 1e9, 2),
+                         ms_per_step=round(wall / steps * 1e3, 5),
+                         kernel_ms=round(kern * 1e3, 4), achieved_GBs=round(ach, 1),
+                         frac=round(ach / HBM_PEAK_GBS, 4), tile_variant=variant)
+        if errors:
+            dus[name] = du.double().cpu().numpy()
+            us[name] = u.double().cpu().numpy()
         del A, u, du, plan
         torch.cuda.empty_cache()
-    return out
+    if not errors or "f64" not in us:
+        return out, None
+    import pyoracle as po
+    ops, bcs = nonuniform3d_oracle_ops(n)
+    # CPU baseline: the oracle (FP64 per-line loops, one temporary per operator) on the FP64
+    # input, one thread, whole applies; then the same on every core this process may use
+    reps, t0 = 0, time.perf_counter()
+    po.set_threads(1)
+    while True:
+        ref64 = po.ghost_apply(ops, bcs, us["f64"])
+        reps += 1
+        el1 = time.perf_counter() - t0
+        if el1 >= cpu_seconds:
+            break
+    thr, host_cores = host_threads()
+    po.set_threads(thr)
+    reps_m, t0 = 0, time.perf_counter()
+    while True:
+        po.ghost_apply(ops, bcs, us["f64"])
+        reps_m += 1
+        el_m = time.perf_counter() - t0
+        if el_m >= max(2.0, cpu_seconds / 3):
+            break
+    for name in ("f64", "f32", "bf16"):
+        if name not in us:
+            continue
+        ref = ref64 if name == "f64" else po.ghost_apply(ops, bcs, us[name])
+        nw, pw = rel_errors(dus[name], ref)
+        out[name]["max_rel_err_vs_f64"] = float(f"{nw:.3e}")
+        out[name]["pointwise_rel_err_vs_f64"] = float(f"{pw:.3e}")
+    po.set_threads(1)
+    cpu = dict(value=round(pts * reps / el1 / 1e9, 5), unit="Gpts/s", cores=1, kind="port",
+               sample=f"{reps} full 512^3 FP64 applies of the oracle (per-line loops of "
+                      f"convolutions.jl, one temporary per operator), {el1:.1f} s, one thread",
+               all_cores=dict(value=round(pts * reps_m / el_m / 1e9, 5), cores=thr,
+                              host_cores=host_cores,
+                              sample=f"{reps_m} applies, {el_m:.1f} s, OpenMP over lines"))
+    return out, cpu
 
 
 # ------------------------------------------------------------------------------------------
@@ -371,7 +457,14 @@ def bench_batched(steps, nprob=1024, plain_sample=48, plain_steps=4):
     _sync()
     plain_s = (time.perf_counter() - t0) / plain_steps
     batched_sample_s = step_s * spts / pts   # batched cost of the same points
+    # algorithmic bytes of one RK4 step: stages read/write 3s, 5s, 5s, 3s bytes per point
+    # (stage 0: u -> acc, s1; 1: s1, u, acc -> acc, s2; 2: s2, u, acc -> acc, s1; 3: s1, acc -> u)
+    step_bytes = 16 * 4 * pts
+    ach = step_bytes / step_s / 1e9
     return dict(
+        roofline=dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                      bytes_per_step=step_bytes),
         problems=nprob, points=pts, rk4_step_ms=round(step_s * 1e3, 4), tile_chunk=rk.chunk,
         rhs_gpts_s=round(4 * pts / step_s / 1e9, 2),
         problem_steps_per_s=round(nprob / step_s, 1),
@@ -419,8 +512,15 @@ def bench_served(nreq=200, lam=2000.0, nsteps=100, budget=8 << 30):
     t = srv.times()
     lat = t[:, 2] - t[:, 0]
     makespan = t[:, 2].max() - t[:, 0].min()
+    # algorithmic bytes: 16 * 4 bytes per point per RK4 step (FP32), over the makespan (which
+    # includes the arrival gaps: a lower bound on the rate while requests are resident)
+    tot_bytes = sum(64.0 * n * n * nsteps for _, n, _, _, _ in reqs)
+    ach = tot_bytes / makespan / 1e9
     served = dict(solves_per_s=round(nreq / makespan, 1), p50_ms=round(np.percentile(lat, 50) * 1e3, 2),
                   p99_ms=round(np.percentile(lat, 99) * 1e3, 2), makespan_s=round(makespan, 3),
+                  roofline=dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                                frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                                bytes_total=tot_bytes),
                   **srv.stats())
     # plain: FIFO, one request at a time
     lat_p, t0 = [], None
@@ -460,17 +560,20 @@ def bench_served(nreq=200, lam=2000.0, nsteps=100, budget=8 << 30):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="laplacian2d",
-                    choices=["laplacian2d", "nonuniform3d", "batched", "served"])
+    ap.add_argument("--config", default="nonuniform3d",
+                    choices=["nonuniform3d", "laplacian2d", "batched", "served"])
     ap.add_argument("--no-also", action="store_true",
-                    help="skip the other workloads (3-D non-uniform, batched RK4, served)")
+                    help="skip the other workloads (2-D Laplacian, batched RK4, served, ...)")
     ap.add_argument("--requests", type=int, default=200, help="served workload: requests")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the oracle legs (CPU baseline and the errors against FP64)")
     ap.add_argument("--copy-reference", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates HBM counters)")
+    ap.add_argument("--types", default="f32,f64,bf16",
+                    help="storage types of the 3-D workload to run (profiling runs pick one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): there is no multi-GPU "
@@ -488,69 +591,82 @@ def main():
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-    if args.config == "laplacian2d":
+    base = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "data": "synthetic"}
+    if args.config == "nonuniform3d":
+        r3, cpu = bench_nonuniform3d(args.steps, args.warmup, errors=not args.no_cpu,
+                                     cpu_seconds=args.cpu_seconds,
+                                     types=tuple(args.types.split(",")))
+        head = "f32" if "f32" in r3 else next(iter(r3))
+        f32 = r3[head]
+        line = dict(base, value=f32["gpts_s"], unit="Gpts/s", steps=args.steps,
+                    warmup=args.warmup, ms_per_step=f32["ms_per_step"], dtype=head)
+        line["data"] = "synthetic (seeded dx, wind and field)"
+        line["config"] = {
+            "workload": "3-D non-uniform 512^3: per axis CenteredDifference{k}(2,6,dx_k,512) + "
+                        "UpwindDifference{k}(1,3,dx_k,512,-v_k), Dirichlet0BC on every face, "
+                        "FP32 storage and accumulation, one (sum L)*Q*u per step",
+            "grid": [512, 512, 512], "storage": head, "accumulation": "f64" if head == "f64" else "f32",
+            "tile_variant": f32["tile_variant"]}
+        line["roofline"] = {"bound": "hbm", "achieved": f32["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": f32["frac"],
+                            "traffic": read_traffic("apply_nonuniform3d_512_f32")}
+        line["max_rel_err_vs_f64"] = f32.get("max_rel_err_vs_f64")
+        line["variants"] = r3
+        if cpu is not None:
+            line["cpu_baseline"] = cpu
+    elif args.config == "laplacian2d":
         r = bench_laplacian2d(args.steps, args.warmup)
-        line = {
-            "metric": METRIC, "value": round(r["value"], 3), "unit": "Gpts/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms_per_step"], 5),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-            "data": "synthetic (seeded uniform [-1,1] field)",
-            "config": {"workload": "2-D Laplacian CenteredDifference{1,2}(2,4,h,4096) + "
-                                   "RobinBC(1,0.5,0) on all edges, 4096x4096 FP64, one Δ*Q*u per step",
-                       "grid": [4096, 4096], "storage": "f64", "accumulation": "f64",
-                       "tile_variant": r["variant"]},
-            "roofline": r["roofline"],
-        }
+        line = dict(base, value=round(r["value"], 3), unit="Gpts/s", steps=args.steps,
+                    warmup=args.warmup, ms_per_step=round(r["ms_per_step"], 5), dtype="f64")
+        line["data"] = "synthetic (seeded uniform [-1,1] field)"
+        line["config"] = {"workload": "2-D Laplacian CenteredDifference{1,2}(2,4,h,4096) + "
+                                      "RobinBC(1,0.5,0) on all edges, 4096x4096 FP64, one Δ*Q*u per step",
+                          "grid": [4096, 4096], "storage": "f64", "accumulation": "f64",
+                          "tile_variant": r["variant"]}
+        line["roofline"] = r["roofline"]
+        if not args.no_cpu:
+            line["cpu_baseline"] = cpu_baseline_laplacian2d(args.cpu_seconds)
     elif args.config == "batched":
         rb = bench_batched(max(10, args.steps // 10))
-        line = {
-            "metric": METRIC, "value": rb["rhs_gpts_s"], "unit": "Gpts/s (RHS evaluations)",
-            "n_gpus": 1, "steps": max(10, args.steps // 10), "warmup": 1,
-            "ms_per_step": rb["rk4_step_ms"], "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "1024 independent 1-D/2-D Δ*Q*u problems, one RK4 step per step"},
-            "batched": rb,
-        }
-    elif args.config == "served":
-        rs = bench_served(args.requests)
-        line = {
-            "metric": METRIC, "value": rs["served"]["solves_per_s"], "unit": "solves/s",
-            "n_gpus": 1, "steps": args.requests, "warmup": 0,
-            "ms_per_step": round(1e3 / max(rs["served"]["solves_per_s"], 1e-9), 3),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic", "config": {"workload": "Poisson-arrival 2-D advection-diffusion "
-                                                        "method-of-lines solves, 100 RK4 steps each"},
-            "served": rs,
-        }
+        line = dict(base, value=rb["rhs_gpts_s"], unit="Gpts/s (RHS evaluations)",
+                    steps=max(10, args.steps // 10), warmup=1, ms_per_step=rb["rk4_step_ms"],
+                    dtype="f32")
+        line["config"] = {"workload": "1024 independent 1-D/2-D Δ*Q*u problems, one RK4 step per step"}
+        line["roofline"] = rb.pop("roofline")
+        line["batched"] = rb
     else:
-        r3 = bench_nonuniform3d(args.steps, args.warmup)
-        f32 = r3["f32"]
-        line = {
-            "metric": METRIC, "value": f32["gpts_s"], "unit": "Gpts/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": f32["kernel_ms"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic", "config": {"workload": "3-D non-uniform 512^3 centered(2,6)+upwind(1,3)"},
-            "roofline": {"bound": "hbm", "achieved": f32["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": f32["frac"], "traffic": None},
-            "variants": r3,
-        }
-    if not args.no_also and args.config == "laplacian2d":
-        line["also"] = {"laplacian2d_4096_f32": bench_laplacian2d_f32(args.steps, args.warmup)}
+        rs = bench_served(args.requests)
+        line = dict(base, value=rs["served"]["solves_per_s"], unit="solves/s", steps=args.requests,
+                    warmup=0, ms_per_step=round(1e3 / max(rs["served"]["solves_per_s"], 1e-9), 3),
+                    dtype="f32")
+        line["config"] = {"workload": "Poisson-arrival 2-D advection-diffusion method-of-lines "
+                                      "solves, 100 RK4 steps each"}
+        line["roofline"] = rs["served"].pop("roofline")
+        line["served"] = rs
+    if not args.no_also and args.config == "nonuniform3d":
+        also = line["also"] = {}
+        r2 = bench_laplacian2d(args.steps, args.warmup)
+        also["laplacian2d_4096_f64"] = dict(
+            gpts_s=round(r2["value"], 2), ms_per_step=round(r2["ms_per_step"], 5),
+            kernel_ms=round(r2["kernel_ms"], 4), roofline=r2["roofline"],
+            tile_variant=r2["variant"])
+        if not args.no_cpu:
+            also["laplacian2d_4096_f64"]["cpu_baseline"] = cpu_baseline_laplacian2d(
+                max(3.0, args.cpu_seconds / 3))
         torch.cuda.empty_cache()
-        line["also"]["nonuniform3d_512"] = bench_nonuniform3d(max(20, args.steps // 4), 3)
+        also["laplacian2d_4096_f32"] = bench_laplacian2d_f32(args.steps, args.warmup)
         torch.cuda.empty_cache()
-        line["also"]["slab_512x512x70_f32"] = bench_slab()
+        also["slab_512x512x70_f32"] = bench_slab()
         torch.cuda.empty_cache()
         try:
-            line["also"]["widened"] = bench_widened()
+            also["widened"] = bench_widened()
         except Exception as e:   # reported, never fatal to the line
-            line["also"]["widened"] = {"error": f"{type(e).__name__}: {e}"}
+            also["widened"] = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()
-        line["also"]["batched_rk4_1024"] = bench_batched(max(10, args.steps // 10))
+        also["batched_rk4_1024"] = bench_batched(max(10, args.steps // 10))
         torch.cuda.empty_cache()
-        line["also"]["served_advdiff"] = bench_served(args.requests)
-    if not args.no_cpu and args.config == "laplacian2d":
-        line["cpu_baseline"] = cpu_baseline_laplacian2d(args.cpu_seconds)
+        also["served_advdiff"] = bench_served(args.requests)
     print(json.dumps(line), flush=True)
 
 

@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,6 +70,125 @@ namespace dop {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 }  // namespace dop
 
+namespace {
+
+// ---- tuning ----
+// Process-wide choices of the tile autotuner, keyed by what decides the candidates and their
+// speed: device, storage, accumulation, grid kind (uniform or not), window, extents.
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune_cache;
+
+std::string tune_key(const dop_plan* p)
+{
+    const dop::PlanDev& P = p->d.host;
+    const dop::FastPlan& F = p->d.fast;
+    std::string k = std::to_string(p->device) + ":" + std::to_string(P.storage) + ":" +
+                    std::to_string(P.accum) + ":" + std::to_string(F.kind) + ":" +
+                    std::to_string(F.args.K) + ":" + std::to_string(F.VX) + ":" +
+                    std::to_string(dop::apply_variants(p->d));
+    for (int a = 0; a < dop::kMaxDim; ++a)
+        k += ":" + std::to_string(P.n[a]) + "/" + std::to_string(F.args.lo[a]) + "/" +
+             std::to_string(F.args.hi[a]) + "/" + std::to_string(F.args.uniform[a]);
+    return k;
+}
+
+bool tune_cache_get(const dop_plan* p, int* v)
+{
+    if (dop::apply_variants(p->d) <= 1) return false;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_tune_cache.find(tune_key(p));
+    if (it == g_tune_cache.end()) return false;
+    *v = it->second;
+    return true;
+}
+
+constexpr int64_t kAutoTuneMinPoints = (int64_t)1 << 20;
+
+// Tune on the first apply: plans with a tiled path of at least 2^20 points, when the stream
+// is not being captured and DOP_AUTOTUNE is not 0.
+bool auto_tune_wanted(const dop_plan* p, hipStream_t s)
+{
+    if (dop::apply_variants(p->d) <= 1 || p->d.host.total < kAutoTuneMinPoints) return false;
+    const char* e = std::getenv("DOP_AUTOTUNE");
+    if (e && e[0] == '0') return false;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
+    return true;
+}
+
+// Times every candidate on (du, u) -- du is overwritten, u only read -- and keeps the fastest in
+// the plan and the cache.  The caller holds p->tune_mu.
+int tune_variants(dop_plan* p, void* du, const void* u, hipStream_t s, int* choice)
+{
+    const int nv = dop::apply_variants(p->d);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return fail(DOP_EHIP, "autotune: event creation failed");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(DOP_EHIP, "autotune: event creation failed");
+    }
+    // steady state, as an application sees it: back-to-back launches of one candidate between
+    // two events; the candidates take turns over three rounds (drift in clocks or
+    // temperature hits them alike) and each keeps its fastest round
+    const char* vb = std::getenv("DOP_AUTOTUNE_VERBOSE");
+    const bool verbose = vb && vb[0] == '1';
+    constexpr int kRounds = 3;
+    std::vector<float> per(nv, 1e30f);
+    std::vector<char> okv(nv, 1);
+    for (int v = 0; v < nv; ++v)   // warm-up (code objects, caches)
+        okv[v] = dop::launch_apply(p->d, du, u, 0, v, 0, s) == hipSuccess;
+    // launches per measurement: enough for about 1 ms of back-to-back work (8 to 64), so that
+    // short kernels are timed in their steady state rather than by their first launches
+    int kReps = 8;
+    {
+        (void)hipEventRecord(e0, s);
+        for (int rep = 0; rep < 4; ++rep) (void)dop::launch_apply(p->d, du, u, 0, 0, 0, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const float t = ms / 4;
+        if (t > 0.0f) kReps = std::max(8, std::min(64, (int)(1.0f / t) + 1));
+    }
+    for (int round = 0; round < kRounds; ++round) {
+        for (int v = 0; v < nv; ++v) {
+            if (!okv[v]) continue;
+            (void)hipEventRecord(e0, s);
+            for (int rep = 0; rep < kReps && okv[v]; ++rep)
+                okv[v] = dop::launch_apply(p->d, du, u, 0, v, 0, s) == hipSuccess;
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            per[v] = std::min(per[v], ms / kReps);
+        }
+    }
+    float best = 1e30f;
+    int best_v = 0;
+    for (int v = 0; v < nv; ++v) {
+        if (verbose)
+            fprintf(stderr, "dop_plan_autotune: variant %d %s %.4f ms\n", v, okv[v] ? "ok" : "failed",
+                    per[v]);
+        if (okv[v] && per[v] < best) {
+            best = per[v];
+            best_v = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGetLastError();   // a candidate that failed to launch is skipped, not reported
+    p->d.variant = best_v;
+    p->tuned.store(1, std::memory_order_release);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[tune_key(p)] = best_v;
+    }
+    if (choice) *choice = best_v;
+    return DOP_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int dop_version(void) { return DOP_ABI_VERSION; }
@@ -78,11 +199,13 @@ int dop_plan_create(const dop_desc* desc, dop_plan** out)
 {
     if (!desc || !out) return fail(DOP_EINVAL, "null argument");
     *out = nullptr;
-    dop::HostPlan H;
+    dop::HostPlan Hb;
     std::string err;
-    const int rc = dop::build_host_plan(*desc, &H, &err);
+    const int rc = dop::build_host_plan(*desc, &Hb, &err);
     if (rc != DOP_OK) return fail(rc, err);
     dop_plan* p = new dop_plan();
+    p->H = std::move(Hb);
+    const dop::HostPlan& H = p->H;
     p->d.host = H.meta;
     p->in_elems = H.in_elems;
     p->out_elems = H.out_elems;
@@ -93,10 +216,18 @@ int dop_plan_create(const dop_desc* desc, dop_plan** out)
     const char* fg = std::getenv("DOP_FORCE_GENERIC");
     p->d.generic_only = (fg && fg[0] == '1') ? 1 : 0;
     if (!p->d.generic_only) dop::build_fast(H, &p->d.fast);
-    // DOP_TILE_VARIANT=v pins the tile-shape candidate (otherwise 0 until dop_plan_autotune)
+    // DOP_TILE_VARIANT=v pins the tile-shape candidate; otherwise the tuning cache's choice
+    // for this shape, or 0 until the plan is tuned
     if (const char* tv = std::getenv("DOP_TILE_VARIANT")) {
         const int v = std::atoi(tv);
         if (v >= 0 && v < dop::apply_variants(p->d)) p->d.variant = v;
+        p->tuned = 1;
+    } else {
+        int v = -1;
+        if (tune_cache_get(p, &v)) {
+            p->d.variant = v;
+            p->tuned = 1;
+        }
     }
     for (int a = 0; a < dop::kMaxDim && r == DOP_OK && p->d.fast.kind != 0; ++a) {
         if ((r = upload(p, p->d.fast.w64[a], &p->d.fast.args.w64[a])) != DOP_OK) break;
@@ -138,7 +269,96 @@ int dop_apply(const dop_plan* plan, void* du, const void* u, int accumulate, voi
 {
     if (!plan || !du || !u) return fail(DOP_EINVAL, "null plan or buffer");
     if (du == u) return fail(DOP_EINVAL, "du must not alias u");
-    DOP_HIP(dop::launch_apply(plan->d, du, u, accumulate ? 1 : 0, -1, 0, (hipStream_t)stream));
+    const hipStream_t s = (hipStream_t)stream;
+    if (!plan->tuned.load(std::memory_order_acquire) && !accumulate && auto_tune_wanted(plan, s)) {
+        // the plan's tuning state is not part of what it computes (every candidate gives the
+        // same values), so a const plan may record its choice
+        dop_plan* mp = const_cast<dop_plan*>(plan);
+        std::lock_guard<std::mutex> lk(mp->tune_mu);
+        if (!mp->tuned.load(std::memory_order_acquire)) {
+            const int rc = tune_variants(mp, du, u, s, nullptr);
+            if (rc != DOP_OK) return rc;
+        }
+    }
+    DOP_HIP(dop::launch_apply(plan->d, du, u, accumulate ? 1 : 0, -1, 0, s));
+    return DOP_OK;
+}
+
+int dop_plan_variant(const dop_plan* plan, int* tuned)
+{
+    if (!plan) return fail(DOP_EINVAL, "null plan");
+    if (tuned) *tuned = plan->tuned.load();
+    return plan->d.variant.load();
+}
+
+int dop_plan_update(dop_plan* plan, const dop_desc* desc, void* stream)
+{
+    if (!plan || !desc) return fail(DOP_EINVAL, "null plan or descriptor");
+    dop::HostPlan H;
+    std::string err;
+    int rc = dop::build_host_plan(*desc, &H, &err);
+    if (rc != DOP_OK) return fail(rc, err);
+    dop::FastPlan F;
+    if (!plan->d.generic_only) dop::build_fast(H, &F);
+    const dop::FastPlan& F0 = plan->d.fast;
+    bool same = dop::same_structure(plan->H.meta, H.meta) && F.kind == F0.kind &&
+                F.args.K == F0.args.K && F.VX == F0.VX && F.shell_total == F0.shell_total;
+    for (int a = 0; a < dop::kMaxDim && same; ++a) {
+        const dop::AxisTables &A0 = plan->H.axis[a], &A1 = H.axis[a];
+        same = A0.w64.size() == A1.w64.size() && A0.w32.size() == A1.w32.size() &&
+               A0.bJ0.size() == A1.bJ0.size() && A0.bn.size() == A1.bn.size() &&
+               A0.bw.size() == A1.bw.size() && A0.lbc.size() == A1.lbc.size() &&
+               F.w64[a].size() == F0.w64[a].size() && F.w32[a].size() == F0.w32[a].size() &&
+               F.args.lo[a] == F0.args.lo[a] && F.args.hi[a] == F0.args.hi[a];
+    }
+    if (!same)
+        return fail(DOP_EINVAL, "dop_plan_update: the descriptor's structure differs from the "
+                                "plan's (create a new plan)");
+    const hipStream_t s = (hipStream_t)stream;
+    // tables whose contents changed: stream-ordered copies, waited for before the host copies
+    // they come from are kept
+    bool copied = false;
+    auto put = [&](const void* dst, const auto& oldv, const auto& newv) -> int {
+        const size_t bytes = newv.size() * sizeof(newv[0]);
+        if (bytes == 0 || !dst || std::memcmp(oldv.data(), newv.data(), bytes) == 0) return DOP_OK;
+        DOP_HIP(hipMemcpyAsync((void*)dst, newv.data(), bytes, hipMemcpyHostToDevice, s));
+        copied = true;
+        return DOP_OK;
+    };
+    for (int a = 0; a < dop::kMaxDim; ++a) {
+        const dop::AxisDev& X = plan->d.host.ax[a];
+        const dop::AxisTables &A0 = plan->H.axis[a], &A1 = H.axis[a];
+        if ((rc = put(X.w64, A0.w64, A1.w64)) != DOP_OK) return rc;
+        if ((rc = put(X.w32, A0.w32, A1.w32)) != DOP_OK) return rc;
+        if ((rc = put(X.bJ0, A0.bJ0, A1.bJ0)) != DOP_OK) return rc;
+        if ((rc = put(X.bn, A0.bn, A1.bn)) != DOP_OK) return rc;
+        if ((rc = put(X.bw, A0.bw, A1.bw)) != DOP_OK) return rc;
+        if ((rc = put(X.lbc, A0.lbc, A1.lbc)) != DOP_OK) return rc;
+        if (F0.kind != 0) {
+            if ((rc = put(F0.args.w64[a], F0.w64[a], F.w64[a])) != DOP_OK) return rc;
+            if ((rc = put(F0.args.w32[a], F0.w32[a], F.w32[a])) != DOP_OK) return rc;
+        }
+    }
+    if (copied) DOP_HIP(hipStreamSynchronize(s));
+    // the plan's own values (BC taps and constants among them): a kernel argument, so the
+    // update is one stream-ordered launch
+    dop::PlanDev v = H.meta;
+    for (int a = 0; a < dop::kMaxDim; ++a) {
+        const dop::AxisDev& X = plan->d.host.ax[a];
+        v.ax[a].w64 = X.w64;
+        v.ax[a].w32 = X.w32;
+        v.ax[a].bJ0 = X.bJ0;
+        v.ax[a].bn = X.bn;
+        v.ax[a].bw = X.bw;
+        v.ax[a].lbc = X.lbc;
+    }
+    DOP_HIP(dop::launch_set_plan(plan->dev, v, s));
+    plan->d.host = v;
+    for (int a = 0; a < dop::kMaxDim; ++a) {
+        plan->d.fast.w64[a] = std::move(F.w64[a]);
+        plan->d.fast.w32[a] = std::move(F.w32[a]);
+    }
+    plan->H = std::move(H);
     return DOP_OK;
 }
 
@@ -146,8 +366,10 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
 {
     if (!plan) return fail(DOP_EINVAL, "null plan");
     const int nv = dop::apply_variants(plan->d);
+    std::lock_guard<std::mutex> lk(plan->tune_mu);
     if (nv <= 1) {
         plan->d.variant = 0;
+        plan->tuned = 1;
         if (choice) *choice = 0;
         return DOP_OK;
     }
@@ -160,63 +382,11 @@ int dop_plan_autotune(dop_plan* plan, void* stream, int* choice)
     }
     hipStream_t s = (hipStream_t)stream;
     (void)hipMemsetAsync(u, 0, plan->in_elems * sb, s);
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    // steady state, as an application sees it: back-to-back launches of one candidate between
-    // two events; the candidates take turns over three rounds (drift in clocks or
-    // temperature hits them alike) and each keeps its fastest round
-    const char* vb = std::getenv("DOP_AUTOTUNE_VERBOSE");
-    const bool verbose = vb && vb[0] == '1';
-    constexpr int kRounds = 3;
-    std::vector<float> per(nv, 1e30f);
-    std::vector<char> okv(nv, 1);
-    for (int v = 0; v < nv; ++v)   // warm-up (code objects, caches)
-        okv[v] = dop::launch_apply(plan->d, du, u, 0, v, 0, s) == hipSuccess;
-    // launches per measurement: enough for about 1 ms of back-to-back work (8 to 64), so that
-    // short kernels are timed in their steady state rather than by their first launches
-    int kReps = 8;
-    {
-        (void)hipEventRecord(e0, s);
-        for (int rep = 0; rep < 4; ++rep) (void)dop::launch_apply(plan->d, du, u, 0, 0, 0, s);
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        const float t = ms / 4;
-        if (t > 0.0f) kReps = std::max(8, std::min(64, (int)(1.0f / t) + 1));
-    }
-    for (int round = 0; round < kRounds; ++round) {
-        for (int v = 0; v < nv; ++v) {
-            if (!okv[v]) continue;
-            (void)hipEventRecord(e0, s);
-            for (int rep = 0; rep < kReps && okv[v]; ++rep)
-                okv[v] = dop::launch_apply(plan->d, du, u, 0, v, 0, s) == hipSuccess;
-            (void)hipEventRecord(e1, s);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            per[v] = std::min(per[v], ms / kReps);
-        }
-    }
-    float best = 1e30f;
-    int best_v = 0;
-    for (int v = 0; v < nv; ++v) {
-        if (verbose)
-            fprintf(stderr, "dop_plan_autotune: variant %d %s %.4f ms\n", v, okv[v] ? "ok" : "failed",
-                    per[v]);
-        if (okv[v] && per[v] < best) {
-            best = per[v];
-            best_v = v;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    const int rc = tune_variants(plan, du, u, s, choice);
+    (void)hipStreamSynchronize(s);
     (void)hipFree(du);
     (void)hipFree(u);
-    plan->d.variant = best_v;
-    if (choice) *choice = best_v;
-    return DOP_OK;
+    return rc;
 }
 
 }  // extern "C"

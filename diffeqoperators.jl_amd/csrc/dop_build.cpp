@@ -161,14 +161,21 @@ int build_host_plan(const dop_desc& D, HostPlan* out, std::string* err)
                 int mnl = 0, mnr = 0;
                 for (int64_t l = 0; l < nlines; ++l) {
                     const dop_affine_bc& e = lb[l];
+                    double* o = T.lbc.data() + (size_t)l * kLineBC;
+                    if (e.kind == DOP_BC_PERIODIC) {
+                        // the line wraps: ghosts u[end], u[1] (one tap each)
+                        o[0] = o[1] = kLinePeriodic;
+                        mnl = std::max(mnl, 1);
+                        mnr = std::max(mnr, 1);
+                        continue;
+                    }
                     if (e.kind != DOP_BC_AFFINE || e.n_l < 0 || e.n_r < 0 ||
                         e.n_l > DOP_MAX_BC_TAPS || e.n_r > DOP_MAX_BC_TAPS || e.n_l > P.n[a] ||
                         e.n_r > P.n[a]) {
                         *err = "per-line boundary condition " + std::to_string(l) +
-                               ": not affine, or more taps than the axis has points";
+                               ": not affine or periodic, or more taps than the axis has points";
                         return DOP_EINVAL;
                     }
-                    double* o = T.lbc.data() + (size_t)l * kLineBC;
                     o[0] = e.n_l;
                     o[1] = e.n_r;
                     for (int k = 0; k < e.n_l; ++k) o[2 + k] = e.a_l[k];
@@ -195,6 +202,27 @@ int build_host_plan(const dop_desc& D, HostPlan* out, std::string* err)
         P.ax[a] = X;
     }
     return DOP_OK;
+}
+
+bool same_structure(const PlanDev& a, const PlanDev& b)
+{
+    if (a.ndim != b.ndim || a.storage != b.storage || a.accum != b.accum || a.ghost != b.ghost ||
+        a.total != b.total)
+        return false;
+    for (int k = 0; k < kMaxDim; ++k) {
+        if (a.n[k] != b.n[k] || a.sdu[k] != b.sdu[k] || a.ssrc[k] != b.ssrc[k] ||
+            a.tshift[k] != b.tshift[k])
+            return false;
+        const AxisDev &x = a.ax[k], &y = b.ax[k];
+        if (x.n != y.n || x.active != y.active || x.K != y.K || x.dmin != y.dmin ||
+            x.rlo != y.rlo || x.rhi != y.rhi || x.uniform != y.uniform || x.KB != y.KB ||
+            x.nb != y.nb || x.nl != y.nl || x.nr != y.nr || x.periodic != y.periodic ||
+            (x.lbc == nullptr) != (y.lbc == nullptr))
+            return false;
+        for (int m = 0; m < kMaxDim; ++m)
+            if (x.lstride[m] != y.lstride[m]) return false;
+    }
+    return true;
 }
 
 }  // namespace dop

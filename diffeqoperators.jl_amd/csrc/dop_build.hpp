@@ -21,4 +21,8 @@ struct HostPlan {
 // Returns DOP_OK or a DOP_E* code with *err set.
 int build_host_plan(const dop_desc& D, HostPlan* out, std::string* err);
 
+// Two plans have the same structure (every field but the BC taps / constants and the table
+// contents and pointers): the launches of one serve the other's values.
+bool same_structure(const PlanDev& a, const PlanDev& b);
+
 }  // namespace dop

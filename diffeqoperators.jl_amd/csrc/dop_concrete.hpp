@@ -41,6 +41,10 @@ DOP_HD inline void ghost_entries(const AxisDev& X, bool left, int64_t base, int6
         return;
     }
     const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
+    if (e && e[0] < 0.0) {   // a periodic line: u[n-1] on the left, u[0] on the right
+        emit(base + (left ? (int64_t)(X.n - 1) * S : 0), w);
+        return;
+    }
     if (left) {
         const int nl = e ? (int)e[0] : X.nl;
         const double* al = e ? e + 2 : X.al;

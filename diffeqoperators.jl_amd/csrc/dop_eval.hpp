@@ -129,33 +129,48 @@ struct Acc<kAccF32Wide> {
 
 // ---------------------------------------------------------------------------------------
 // ghost values of one line: a . u[0:nl] + b (bc_operators.jl:188-191), with the line's own
-// BC when the axis has one per line (multi_dim_bc_operators.jl:34, 93-95)
+// BC when the axis has one per line (multi_dim_bc_operators.jl:34, 93-95); a periodic line
+// wraps (u[n-1] on the left, u[0] on the right, bc_operators.jl:192).  ld(j) is element j of
+// the line, as a double.
+template <typename F>
+DOP_HD inline double ghost_left_f(const AxisDev& X, int64_t line, const F& ld)
+{
+    if (X.periodic) return ld(X.periodic == 1 ? (int64_t)(X.n - 1) : 0);
+    const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
+    if (e && e[0] < 0.0) return ld((int64_t)(X.n - 1));
+    const int nl = e ? (int)e[0] : X.nl;
+    const double* al = e ? e + 2 : X.al;
+    double g = 0.0;
+    for (int k = 0; k < nl; ++k) g = fma(al[k], ld(k), g);
+    return g + (e ? e[2 + kMaxBCTaps] : X.bl);
+}
+
+template <typename F>
+DOP_HD inline double ghost_right_f(const AxisDev& X, int64_t line, const F& ld)
+{
+    if (X.periodic) return ld(X.periodic == 2 ? (int64_t)(X.n - 1) : 0);
+    const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
+    if (e && e[1] < 0.0) return ld(0);
+    const int nr = e ? (int)e[1] : X.nr;
+    const double* ar = e ? e + 3 + kMaxBCTaps : X.ar;
+    double g = 0.0;
+    const int64_t o = (int64_t)X.n - nr;
+    for (int k = 0; k < nr; ++k) g = fma(ar[k], ld(o + k), g);
+    return g + (e ? e[3 + 2 * kMaxBCTaps] : X.br);
+}
+
 template <typename T>
 DOP_HD inline double ghost_left(const AxisDev& X, const T* src, int64_t base, int64_t S,
                                 int64_t line)
 {
-    if (X.periodic) return (double)Store<T>::ld(src, base + (X.periodic == 1 ? (int64_t)(X.n - 1) * S : 0));
-    const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
-    const int nl = e ? (int)e[0] : X.nl;
-    const double* al = e ? e + 2 : X.al;
-    double g = 0.0;
-    for (int k = 0; k < nl; ++k) g = fma(al[k], (double)Store<T>::ld(src, base + k * S), g);
-    return g + (e ? e[2 + kMaxBCTaps] : X.bl);
+    return ghost_left_f(X, line, [&](int64_t j) { return (double)Store<T>::ld(src, base + j * S); });
 }
 
 template <typename T>
 DOP_HD inline double ghost_right(const AxisDev& X, const T* src, int64_t base, int64_t S,
                                  int64_t line)
 {
-    if (X.periodic) return (double)Store<T>::ld(src, base + (X.periodic == 2 ? (int64_t)(X.n - 1) * S : 0));
-    const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
-    const int nr = e ? (int)e[1] : X.nr;
-    const double* ar = e ? e + 3 + kMaxBCTaps : X.ar;
-    double g = 0.0;
-    const int64_t o = (int64_t)X.n - nr;
-    for (int k = 0; k < nr; ++k)
-        g = fma(ar[k], (double)Store<T>::ld(src, base + (o + k) * S), g);
-    return g + (e ? e[3 + 2 * kMaxBCTaps] : X.br);
+    return ghost_right_f(X, line, [&](int64_t j) { return (double)Store<T>::ld(src, base + j * S); });
 }
 
 // line index of output point idx along axis a (per-line BCs only)
@@ -167,10 +182,11 @@ DOP_HD inline int64_t line_of(const AxisDev& X, const int64_t* idx)
 constexpr int kEvalUnroll = 8;
 
 // FP64 value of a boundary row (always evaluated in double).  The ghosts a row touches are
-// formed once, and every tap's load is issued before the first FMA.
-template <typename T>
-DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* src, int r,
-                                  int64_t base, int64_t S, int64_t line)
+// formed once, and every tap's load is issued before the first FMA.  ld(j) is element j of
+// the source line (u, or the padded M), as a double.
+template <typename F>
+DOP_HD inline double boundary_row_f(const PlanDev& P, const AxisDev& X, int r, int64_t line,
+                                    const F& ld)
 {
     const int b = r < X.rlo ? r : X.rlo + (r - X.rhi);
     const int J0 = X.bJ0[b];
@@ -179,8 +195,8 @@ DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* s
     const int joff = P.ghost ? -1 : 0;
     const bool gl_used = P.ghost && J0 == 0;
     const bool gr_used = P.ghost && J0 + nt - 1 == X.n + 1;
-    const double gl = gl_used ? ghost_left<T>(X, src, base, S, line) : 0.0;
-    const double gr = gr_used ? ghost_right<T>(X, src, base, S, line) : 0.0;
+    const double gl = gl_used ? ghost_left_f(X, line, ld) : 0.0;
+    const double gr = gr_used ? ghost_right_f(X, line, ld) : 0.0;
     double s = 0.0;
     if (nt <= kEvalUnroll) {
         double v[kEvalUnroll];
@@ -190,7 +206,7 @@ DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* s
                 const int J = J0 + k;
                 if (gl_used && J == 0) v[k] = gl;
                 else if (gr_used && J == X.n + 1) v[k] = gr;
-                else v[k] = (double)Store<T>::ld(src, base + (int64_t)(J + joff) * S);
+                else v[k] = ld((int64_t)(J + joff));
             }
         }
 #pragma unroll
@@ -203,10 +219,18 @@ DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* s
         double v;
         if (gl_used && J == 0) v = gl;
         else if (gr_used && J == X.n + 1) v = gr;
-        else v = (double)Store<T>::ld(src, base + (int64_t)(J + joff) * S);
+        else v = ld((int64_t)(J + joff));
         s = fma(w[k], v, s);
     }
     return s;
+}
+
+template <typename T>
+DOP_HD inline double boundary_row(const PlanDev& P, const AxisDev& X, const T* src, int r,
+                                  int64_t base, int64_t S, int64_t line)
+{
+    return boundary_row_f(P, X, r, line,
+                          [&](int64_t j) { return (double)Store<T>::ld(src, base + j * S); });
 }
 
 // The terms of axis X (row r, line origin `base`, stride S) added to acc: a regular row's K

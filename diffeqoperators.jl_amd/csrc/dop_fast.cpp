@@ -114,6 +114,24 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     }
     // shells: the domain minus the box, slabs from the slowest axis down
     fp->shell_total = shell_boxes(P, A.lo, A.hi, &fp->shell);
+    {
+        StripArgs& S = fp->strips;
+        S = StripArgs{};
+        S.nbox = fp->shell.nbox;
+        S.vec_ok = (P.n[0] % kStripPts == 0) ? 1 : 0;
+        int64_t tot = 0;
+        for (int b = 0; b < S.nbox; ++b) {
+            for (int k = 0; k < 3; ++k) {
+                S.org[b][k] = fp->shell.org[b][k];
+                S.ext[b][k] = fp->shell.ext[b][k];
+            }
+            S.nsx[b] = ceil_div(S.ext[b][0], kStripPts);
+            S.xcol[b] = S.ext[b][0] < P.n[0] ? 1 : 0;
+            S.start[b] = tot;
+            tot += S.nsx[b] * S.ext[b][1] * S.ext[b][2];
+        }
+        S.start[S.nbox] = tot;
+    }
     (void)shell_boxes(P, fp->nat_lo, fp->nat_hi, &fp->nat_shell);
     fp->kind = P.ndim;
 }
@@ -124,13 +142,16 @@ void build_fast(const HostPlan& H, FastPlan* fp)
 // {widest, half}.  A block stages waves * rows y rows and computes all but the 2H halo rows.
 namespace {
 bool wide_ok(const FastPlan& fp) { return fp.VX > 1 && fp.args.n[0] % (2 * fp.VX) == 0; }
+
+int gen1_count(const FastPlan& fp) { return fp.VX > 1 ? 40 : 20; }
+int gen2_count(const FastPlan& fp) { return fp.VX > 1 ? 32 : 16; }
 }  // namespace
 
 int fast_variant_count(const FastPlan& fp)
 {
     if (fp.kind == 1) return 1;
     if (fp.kind == 2) return wide_ok(fp) ? 32 : 16;
-    if (fp.kind == 3) return fp.VX > 1 ? 40 : 20;
+    if (fp.kind == 3) return gen1_count(fp) + gen2_count(fp);
     return 1;
 }
 
@@ -138,6 +159,28 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
 {
     if (fp.kind < 2 || v < 0 || v >= fast_variant_count(fp)) return false;
     FastArgs A = fp.args;
+    A.gen = 1;
+    if (fp.kind == 3 && v >= gen1_count(fp)) {
+        // edge-loaded segments: chunk depth {64,32} x block shape (waves, rows per wave)
+        // {(8,1),(8,2),(16,1),(16,2)} x XCD remap x points per lane {widest, half}
+        const int w = v - gen1_count(fp);
+        A.gen = 2;
+        A.xcd_remap = (w & 1) ? 0 : 1;
+        static const int chunks[2] = {64, 32};
+        A.chunk = chunks[(w >> 1) & 1];
+        static const int shapes[4][2] = {{8, 1}, {8, 2}, {16, 1}, {16, 2}};
+        A.W = shapes[(w >> 2) & 3][0];
+        A.VY = shapes[(w >> 2) & 3][1];
+        A.VX = (w >> 4) ? fp.VX / 2 : fp.VX;
+        A.PF = 1;
+        A.xbase = A.lo[0];
+        A.nxseg = (int)ceil_div(A.hi[0] - A.lo[0], 64 * A.VX);
+        A.nyblk = (int)ceil_div(A.hi[1] - A.lo[1], A.W * A.VY);
+        A.gx = 0;
+        A.nblocks = (int)(A.nxseg * A.nyblk * ceil_div(A.hi[2] - A.lo[2], A.chunk));
+        *out = A;
+        return true;
+    }
     A.xcd_remap = (v & 1) ? 0 : 1;
     if (fp.kind == 2) {
         static const int chunks[4] = {32, 16, 64, 8};

@@ -20,6 +20,19 @@ struct ShellArgs {
     int64_t start[kMaxBoxes + 1];   // prefix counts; start[nbox] = total
 };
 
+// The shell as strips of up to kStripPts consecutive x points of one row (the 3-D shell
+// kernel's work items): per box, its origin, extent, strips along x and prefix counts.
+constexpr int kStripPts = 4;
+struct StripArgs {
+    int32_t nbox;
+    int32_t vec_ok;                 // n0 % 4 == 0: strips starting at x % 4 == 0 are aligned
+    int64_t org[kMaxBoxes][3];
+    int64_t ext[kMaxBoxes][3];
+    int64_t nsx[kMaxBoxes];         // strips along x
+    int64_t start[kMaxBoxes + 1];   // prefix counts of strips; start[nbox] = total
+    int32_t xcol[kMaxBoxes];        // 1: the box holds x boundary columns (stage the line end)
+};
+
 // The interior box and the per-axis weights of the tiled kernels.  Every active axis uses
 // one symmetric odd window of K taps (narrower operator windows are padded with zero
 // weights); rows/planes march with a register ring.
@@ -40,6 +53,8 @@ struct FastArgs {
     int32_t gx;            // 2-D: blocks across x (4 segments each)
     int32_t nblocks;
     int32_t xcd_remap;     // keep neighbouring tiles on one XCD
+    int32_t gen;           // 3-D: 1 overlapping segments (x halo from the segment's own end
+                           // lanes), 2 edge-loaded segments (k_tiled3e)
     int32_t uniform[3];    // weights identical for every row of the axis
     const double* w64[3];  // [uniform ? 1 : n][K]
     const float* w32[3];
@@ -53,6 +68,7 @@ struct FastPlan {
     FastArgs args{};
     ShellArgs shell{};
     int64_t shell_total = 0;
+    StripArgs strips{};    // the same shell as strips (3-D)
     // the box before the x edges are aligned for the single-plan kernels' vector stores, and
     // its shell (the batched tiles store single points and use these)
     int32_t nat_lo[3] = {0, 0, 0}, nat_hi[3] = {1, 1, 1};

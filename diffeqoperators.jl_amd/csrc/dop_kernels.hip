@@ -158,6 +158,21 @@ static hipError_t launch_generic(const PlanDev& host, const PlanDev* dev, void* 
 
 int apply_variants(const DevicePlan& p) { return p.fast.kahan ? 1 : fast_variant_count(p.fast); }
 
+// copies a plan's values, passed by value, word by word into its device copy
+__global__ __launch_bounds__(64) void k_set_plan(PlanDev* __restrict__ dst, const PlanDev v)
+{
+    static_assert(sizeof(PlanDev) % sizeof(uint64_t) == 0, "whole words");
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&v);
+    uint64_t* out = reinterpret_cast<uint64_t*>(dst);
+    for (int i = threadIdx.x; i < (int)(sizeof(PlanDev) / sizeof(uint64_t)); i += 64) out[i] = src[i];
+}
+
+hipError_t launch_set_plan(PlanDev* dst, const PlanDev& v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_plan, dim3(1), dim3(64), 0, s, dst, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_tiled(const DevicePlan& p, void* du, const void* u, int accumulate, int variant,
                         hipStream_t s);
 
@@ -191,7 +206,8 @@ hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumu
     // the tiled kernels use 16-byte lane loads/stores: both arrays must be 16-byte aligned
     const bool aligned = ((uintptr_t)du % 16 == 0) && ((uintptr_t)u % 16 == 0);
     if (p.fast.kind >= 2 && !p.fast.kahan && !force_generic && aligned)
-        return launch_tiled(p, du, u, accumulate, variant < 0 ? p.variant : variant, s);
+        return launch_tiled(p, du, u, accumulate,
+                            variant < 0 ? p.variant.load(std::memory_order_relaxed) : variant, s);
     return launch_generic_any(p.host, p.dev, du, u, accumulate, s,
                               !force_generic && !p.generic_only);
 }

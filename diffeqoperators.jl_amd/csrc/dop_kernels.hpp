@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "dop_fast.hpp"
 #include "dop_plan.hpp"
 
@@ -13,7 +15,7 @@ struct DevicePlan {
     PlanDev host{};
     const PlanDev* dev = nullptr;
     FastPlan fast;
-    int variant = 0;   // tile-shape candidate chosen by the autotuner
+    std::atomic<int> variant{0};   // tile-shape candidate chosen by the autotuner
     int generic_only = 0;   // every point on k_apply_generic (no tiles, no single-axis kernel)
 };
 
@@ -28,5 +30,8 @@ hipError_t launch_apply(const DevicePlan& p, void* du, const void* u, int accumu
 
 // Number of tile-shape candidates of the plan (autotuner).
 int apply_variants(const DevicePlan& p);
+
+// *dst = v on stream s (the new values of a plan, carried by the launch itself).
+hipError_t launch_set_plan(PlanDev* dst, const PlanDev& v, hipStream_t s);
 
 }  // namespace dop

@@ -2,11 +2,14 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/dop.h"
 #include "dop_batch.hpp"
+#include "dop_build.hpp"
 #include "dop_kernels.hpp"
 
 namespace dop {
@@ -17,6 +20,9 @@ int set_error(int code, const std::string& msg);
 struct dop_plan {
     dop::DevicePlan d;            // host copy (device table pointers), device copy, tiled path
     dop::PlanDev* dev = nullptr;  // device copy of d.host
+    dop::HostPlan H;              // host tables of the current values (dop_plan_update)
+    std::atomic<int> tuned{0};    // variant chosen by the autotuner (or the tuning cache)
+    std::mutex tune_mu;           // one tuning at a time
     std::vector<void*> bufs;
     int64_t in_elems = 0, out_elems = 0;
     int device = 0;

@@ -48,13 +48,15 @@ struct AxisDev {
     const int32_t* bn;    // [nb] taps
     const double* bw;     // [nb][KB]
     // per-line BCs (nullptr: al/bl/ar/br above): line l at lbc + l * kLineBC, laid out as
-    // nl, nr, al[8], bl, ar[8], br; line of a point = sum over the other axes b of idx[b] *
-    // lstride[b]
+    // nl, nr, al[8], bl, ar[8], br (nl = nr = kLinePeriodic: the line wraps, ghosts u[n-1],
+    // u[0]); line of a point = sum over the other axes b of idx[b] * lstride[b]
     const double* lbc;
     int64_t lstride[kMaxDim];
 };
 
 constexpr int kLineBC = 2 * kMaxBCTaps + 4;
+// nl (and nr) of a per-line entry whose line wraps (PeriodicBC on that line)
+constexpr double kLinePeriodic = -1.0;
 
 struct PlanDev {
     int32_t ndim;

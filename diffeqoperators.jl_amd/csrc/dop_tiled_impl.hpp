@@ -26,6 +26,7 @@
 #include "dop_eval.hpp"
 #include "dop_kernels.hpp"
 #include "dop_ring.hpp"
+#include "dop_shell.hpp"
 
 namespace dop {
 namespace tiled {
@@ -623,15 +624,323 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     }
 }
 
-// The shell of a 3-D plan: one point per thread in small blocks.  This per-point work is
-// latency-bound; in its own launch it runs at full occupancy instead of inheriting the tile
-// kernel's 1024-thread blocks, registers and LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void k_shell(const PlanDev* __restrict__ Pp, const ShellArgs SH,
-                                             T* __restrict__ du, const T* __restrict__ u,
-                                             int accumulate)
+// ---------------------------------------------------------------------------------------
+// 3-D, edge-loaded form.  Segments do not overlap: a wave owns an aligned 64*VX-point x
+// segment of VY consecutive rows and every lane computes.  The x window comes from the
+// neighbouring lanes through DPP; the outermost lanes take the NE groups beyond the segment
+// from LDS (lane 0 those on the left, lane 63 those on the right, as the DPP's fill value), where
+// a few lanes of every wave stage them next to the segment each step.  The block's W*VY rows
+// plus H halo rows above and below are staged in LDS (double-buffered, one barrier per
+// step); y taps are LDS reads, z taps come from a register ring of K + PF planes per
+// computed row.  Halo rows and edge groups are loaded one plane ahead into two-slot queues
+// (the ring length K + PF is even, so the slot of a plane is its parity within the ring).
+// Weights along y are wave-uniform (scalar registers), along z block-uniform (LDS), along x
+// per column (registers, or LDS when VX * K values would not fit).  The per-point FMA
+// sequence is the generic evaluator's: x taps, then y, then z, each in tap order.
+template <typename C>
+__device__ inline C uniform_val(C v);
+template <>
+__device__ inline float uniform_val<float>(float v)
 {
-    shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)blockIdx.x * 256, 256);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+template <>
+__device__ inline double uniform_val<double>(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// lane i <- lane i-1 (lane 0 keeps `fill`) / lane i <- lane i+1 (lane 63 keeps `fill`)
+__device__ inline float dpp_shr1_fill(float fill, float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                                  __builtin_bit_cast(int, v),
+                                                                  0x138, 0xf, 0xf, false));
+}
+__device__ inline float dpp_shl1_fill(float fill, float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                                  __builtin_bit_cast(int, v),
+                                                                  0x130, 0xf, 0xf, false));
+}
+__device__ inline double dpp2(double fill, double v, int ctrl_is_shl)
+{
+    const uint64_t f = __builtin_bit_cast(uint64_t, fill), b = __builtin_bit_cast(uint64_t, v);
+    int lo, hi;
+    if (ctrl_is_shl) {
+        lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, 0x130, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), 0x130,
+                                         0xf, 0xf, false);
+    } else {
+        lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, 0x138, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), 0x138,
+                                         0xf, 0xf, false);
+    }
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ inline double dpp_shr1_fill(double fill, double v) { return dpp2(fill, v, 0); }
+__device__ inline double dpp_shl1_fill(double fill, double v) { return dpp2(fill, v, 1); }
+
+// x window xw[e] = u(xs - H + e), e < VX + 2H, of a lane's VX points: its own values, its
+// neighbours' through DPP, and at the wave's ends the edge groups eg[g] (lane 0: group
+// -(g+1) of the segment, lane 63: group 64+g).
+template <typename C, int VX, int H>
+__device__ inline void x_window_edges(const C* own, const C (*eg)[VX], C* xw)
+{
+    constexpr int NE = (H + VX - 1) / VX;
+#pragma unroll
+    for (int v = 0; v < VX; ++v) xw[H + v] = own[v];
+    C l[VX], r[VX];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+        l[v] = own[v];
+        r[v] = own[v];
+    }
+#pragma unroll
+    for (int d = 1; d <= NE; ++d) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+            const int el = H - d * VX + v;              // u(xs - d*VX + v)
+            const int er = H + VX + (d - 1) * VX + v;   // u(xs + d*VX + v)
+            // value v of the group at distance d lands in the window (and then so does every
+            // nearer one, so the shift chain of v runs up to the farthest such d)
+            if (el >= 0) {
+                l[v] = dpp_shr1_fill(eg[d - 1][v], l[v]);
+                xw[el] = l[v];
+            }
+            if (er < VX + 2 * H) {
+                r[v] = dpp_shl1_fill(eg[d - 1][v], r[v]);
+                xw[er] = r[v];
+            }
+        }
+    }
+}
+
+template <typename T, int VX, int K, bool UNI, int PF, int W, int VY>
+__global__ __launch_bounds__(64 * W) void k_tiled3e(const FastArgs A, T* __restrict__ du,
+                                                     const T* __restrict__ u, int accumulate)
+{
+    using C = typename Store<T>::C;
+    constexpr int H = K / 2;
+    constexpr int R = K + PF;
+    static_assert(R % 2 == 0, "two-slot queues advance with an even ring");
+    static_assert(2 * H <= W, "one halo row per wave at most");
+    constexpr int TY = W * VY;                          // computed rows per block
+    constexpr int SR = TY + 2 * H;                      // staged rows
+    constexpr int NE = (H + VX - 1) / VX;               // edge groups on each side
+    constexpr int ROWE = (64 + 2 * NE) * VX;            // staged row: edges + segment + edges
+    constexpr int C0 = NE * VX;                         // column of the segment's first point
+    constexpr bool XWL = !UNI && VX * K * sizeof(C) > 64;   // x weights in LDS
+    static_assert(2 * NE * VY <= 64, "edge groups of a wave's rows fit its lanes");
+    __shared__ __attribute__((aligned(16))) C lds[2][SR][ROWE];
+    __shared__ C wzs[UNI ? 1 : kMaxChunk3D][K];
+    __shared__ __attribute__((aligned(16))) C wxs[XWL ? K : 1][XWL ? 64 * VX : 1];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b = tile_of((int)blockIdx.x, A.nblocks, A.xcd_remap);
+    // logical tile: y block fastest (neighbouring y blocks, which read each other's rows as
+    // halo, land on one XCD), then x segment, then z chunk
+    const int yblk = b % A.nyblk;
+    const int t = b / A.nyblk;
+    const int xseg = t % A.nxseg;
+    const int zch = t / A.nxseg;
+    const int y0 = A.lo[1] + yblk * TY;
+    const int z0 = A.lo[2] + zch * A.chunk;
+    const int z1 = min(z0 + A.chunk, A.hi[2]);
+    const int64_t n0 = A.n[0];
+    const int n1 = (int)A.n[1], n2 = (int)A.n[2];
+    const int64_t s1 = A.s1, s2 = A.s2;
+    const int64_t xs0 = (int64_t)A.lo[0] + (int64_t)xseg * (64 * VX);
+    const int64_t xs = xs0 + lane * VX;
+    // Loads are clamped into the array: values of clamped lanes, rows or planes are only used
+    // by outputs that are not stored (an output in the box [lo, hi) reads no further than H
+    // points past the box, inside the array).
+    const int64_t xl = min(xs, n0 - VX);
+    const bool xst = xs < A.hi[0];
+    // edge duty: lane e < 2*NE*VY stages group g of side sd of computed row ei
+    const int ei = lane / (2 * NE), sd = (lane / NE) & 1, eg_g = lane % NE;
+    const bool edge_lane = lane < 2 * NE * VY;
+    int64_t xe = sd == 0 ? xs0 - (int64_t)(NE - eg_g) * VX : xs0 + (int64_t)(64 + eg_g) * VX;
+    xe = edge_lane ? min(max(xe, (int64_t)0), n0 - VX) : xl;
+    const int ecol = sd == 0 ? eg_g * VX : C0 + 64 * VX + eg_g * VX;
+    // fill values of the DPP window: lane 0 reads groups -(g+1), lane 63 groups 64+g
+    const int fcol0 = lane == 0 ? (NE - 1) * VX : C0 + 64 * VX;   // column of g = 0
+    const int fstep = lane == 0 ? -VX : VX;
+
+    int rr[VY], y[VY], yc[VY];
+    bool st_ok[VY], live[VY];
+#pragma unroll
+    for (int i = 0; i < VY; ++i) {
+        rr[i] = H + wave * VY + i;
+        y[i] = y0 + wave * VY + i;
+        yc[i] = min(y[i], n1 - 1);
+        live[i] = y[i] < A.hi[1];                       // wave-uniform
+        st_ok[i] = live[i] && xst;
+    }
+    const int jh = wave;                                 // halo row of this wave, if < 2H
+    const bool has_halo = jh < 2 * H;
+    const int hrow = jh < H ? jh : TY + jh;
+    const int yh = min(max(y0 - H + hrow, 0), n1 - 1);
+    const int erow = H + wave * VY + min(ei, VY - 1);    // staged row of the edge group
+    const int ye = min(y0 + wave * VY + min(ei, VY - 1), n1 - 1);
+
+    const C* wxt = wtab<C>(A, 0);
+    const C* wyt = wtab<C>(A, 1);
+    const C* wzt = wtab<C>(A, 2);
+    C wx[XWL ? 1 : VX][K];
+    C wy[VY][K];
+    C wxu[K], wzu[K];
+#pragma unroll
+    for (int i = 0; i < VY; ++i) {
+        const int64_t yr = UNI ? 0 : yc[i];
+#pragma unroll
+        for (int k = 0; k < K; ++k) wy[i][k] = uniform_val<C>(wyt[yr * K + k]);
+    }
+    if constexpr (UNI) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            wxu[k] = uniform_val<C>(wxt[k]);
+            wzu[k] = uniform_val<C>(wzt[k]);
+        }
+    } else {
+        for (int i = threadIdx.x; i < (z1 - z0) * K; i += 64 * W)
+            wzs[i / K][i % K] = wzt[(int64_t)(z0 + i / K) * K + i % K];
+        if constexpr (XWL) {
+            for (int i = threadIdx.x; i < 64 * VX * K; i += 64 * W) {
+                const int c = i / K, k = i % K;
+                const int64_t x = min(xs0 + c, n0 - 1);
+                wxs[k][c] = wxt[x * K + k];
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < VX; ++v) {
+                const int64_t xr = min(xs + v, n0 - 1);
+#pragma unroll
+                for (int k = 0; k < K; ++k) wx[v][k] = wxt[xr * K + k];
+            }
+        }
+    }
+
+    const T* urow[VY];
+#pragma unroll
+    for (int i = 0; i < VY; ++i) urow[i] = u + (int64_t)yc[i] * s1 + xl;
+    const T* uhalo = u + (int64_t)yh * s1 + xl;
+    const T* uedge = u + (int64_t)ye * s1 + xe;
+    const int zlast = min(z1 + H - 1, n2 - 1);
+    auto zc = [&](int zz) { return (int64_t)min(max(zz, 0), zlast) * s2; };
+
+    C q[VY][R][VX];
+    C hq[2][VX];   // halo row, planes z and z + 1
+    C eq[2][VX];   // edge group, planes z and z + 1
+#pragma unroll
+    for (int s = 0; s < R - 1; ++s) {
+#pragma unroll
+        for (int i = 0; i < VY; ++i) VecIO<T, VX>::ld(urow[i] + zc(z0 - H + s), q[i][s]);
+    }
+    VecIO<T, VX>::ld(uhalo + zc(z0), hq[0]);
+    VecIO<T, VX>::ld(uedge + zc(z0), eq[0]);
+
+    int buf = 0;
+    for (int zb = z0; zb < z1; zb += R) {
+        auto step = [&](auto jc) -> bool {
+            constexpr int j = decltype(jc)::value;
+            constexpr int cur = j & 1, nxt = (j + 1) & 1;
+            const int z = zb + j;
+            if (z >= z1) return false;   // block-uniform
+#pragma unroll
+            for (int i = 0; i < VY; ++i)
+                VecIO<T, VX>::ld(urow[i] + zc(z + H + PF), q[i][(j + K - 1 + PF) % R]);
+            VecIO<T, VX>::ld(uhalo + zc(z + 1), hq[nxt]);
+            VecIO<T, VX>::ld(uedge + zc(z + 1), eq[nxt]);
+            // stage plane z: computed rows, the halo row, the edge groups
+#pragma unroll
+            for (int i = 0; i < VY; ++i) {
+#pragma unroll
+                for (int v = 0; v < VX; ++v) lds[buf][rr[i]][C0 + lane * VX + v] = q[i][(j + H) % R][v];
+            }
+            if (has_halo) {
+#pragma unroll
+                for (int v = 0; v < VX; ++v) lds[buf][hrow][C0 + lane * VX + v] = hq[cur][v];
+            }
+            if (edge_lane) {
+#pragma unroll
+                for (int v = 0; v < VX; ++v) lds[buf][erow][ecol + v] = eq[cur][v];
+            }
+            __syncthreads();
+            C wz[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if constexpr (UNI) wz[k] = wzu[k];
+                else wz[k] = wzs[z - z0][k];
+            }
+            C wxl[XWL ? K : 1][XWL ? VX : 1];
+            if constexpr (XWL) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+#pragma unroll
+                    for (int v = 0; v < VX; ++v) wxl[k][v] = wxs[k][lane * VX + v];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < VY; ++i) {
+                if (!live[i]) continue;   // wave-uniform
+                const C* c = q[i][(j + H) % R];
+                C eg[NE][VX];
+#pragma unroll
+                for (int g = 0; g < NE; ++g) {
+#pragma unroll
+                    for (int v = 0; v < VX; ++v) eg[g][v] = C(0);
+                }
+                if (lane == 0 || lane == 63) {
+#pragma unroll
+                    for (int g = 0; g < NE; ++g) {
+#pragma unroll
+                        for (int v = 0; v < VX; ++v) eg[g][v] = lds[buf][rr[i]][fcol0 + g * fstep + v];
+                    }
+                }
+                C xw[VX + 2 * H];
+                x_window_edges<C, VX, H>(c, eg, xw);
+                C out[VX];
+#pragma unroll
+                for (int v = 0; v < VX; ++v) {
+                    C s = C(0);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        C w;
+                        if constexpr (UNI) w = wxu[k];
+                        else if constexpr (XWL) w = wxl[k][v];
+                        else w = wx[v][k];
+                        s = fmac<C>(w, xw[v + k], s);
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const C val = k == H ? c[v] : lds[buf][rr[i] - H + k][C0 + lane * VX + v];
+                        s = fmac<C>(wy[i][k], val, s);
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[i][(j + k) % R][v], s);
+                    out[v] = s;
+                }
+                if (st_ok[i]) {
+                    T* po = du + (int64_t)z * s2 + (int64_t)y[i] * s1 + xs;
+                    if (accumulate) {
+                        C old[VX];
+                        VecIO<T, VX>::ld(po, old);
+#pragma unroll
+                        for (int v = 0; v < VX; ++v) out[v] = (C)((double)out[v] + (double)old[v]);
+                    }
+                    VecIO<T, VX>::st(po, out);
+                }
+            }
+            buf ^= 1;
+            return true;
+        };
+        if (!ring_steps<0, R>(step)) break;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -648,10 +957,35 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
     // 2-D: the (thin) shell rides in the tile launch; 3-D: a launch of its own first
     const int nshell = fp.kind == 2 ? (int)((fp.shell_total + per - 1) / per) : 0;
     if (fp.kind == 3 && fp.shell_total > 0) {
-        hipLaunchKernelGGL((k_shell<T>), dim3((unsigned)((fp.shell_total + 255) / 256)), dim3(256), 0,
-                           s, p.dev, fp.shell, (T*)du, (const T*)u, accumulate);
+        const int64_t items = fp.strips.start[fp.strips.nbox];
+        hipLaunchKernelGGL((shell::k_shell_strips<T, acc_of<T>()>),
+                           dim3((unsigned)((items + shell::kShellBlock - 1) / shell::kShellBlock)),
+                           dim3(shell::kShellBlock), 0, s, p.dev, fp.strips, (T*)du, (const T*)u,
+                           accumulate);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (fp.kind == 3 && A.gen == 2) {
+        if constexpr (K <= 9 && VX <= 16 / (int)sizeof(T) && VX <= 4) {
+#define DOP_L3E(UNI, W, VY)                                                                  \
+    hipLaunchKernelGGL((k_tiled3e<T, VX, K, UNI, 1, W, VY>), dim3((unsigned)A.nblocks),        \
+                       dim3(64 * W), 0, s, A, (T*)du, (const T*)u, accumulate)
+#define DOP_L3EU(W, VY)                                                                      \
+    do {                                                                                     \
+        if (uni) DOP_L3E(true, W, VY); else DOP_L3E(false, W, VY);                           \
+    } while (0)
+            const int shape = A.W * 8 + A.VY;
+            if (shape == 8 * 8 + 1) DOP_L3EU(8, 1);
+            else if (shape == 8 * 8 + 2) DOP_L3EU(8, 2);
+            else if (shape == 16 * 8 + 1) DOP_L3EU(16, 1);
+            else if (shape == 16 * 8 + 2) DOP_L3EU(16, 2);
+            else return hipErrorInvalidValue;
+#undef DOP_L3EU
+#undef DOP_L3E
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
     }
     const dim3 grid((unsigned)(A.nblocks + nshell));
     if (fp.kind == 2) {

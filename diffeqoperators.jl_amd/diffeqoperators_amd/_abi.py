@@ -87,6 +87,8 @@ def lib():
     L.dop_plan_output_elems.restype = C.c_int64
     L.dop_apply.argtypes = [vp, vp, vp, i, vp]
     L.dop_plan_autotune.argtypes = [vp, vp, C.POINTER(i)]
+    L.dop_plan_update.argtypes = [vp, C.POINTER(DopDesc), vp]
+    L.dop_plan_variant.argtypes = [vp, C.POINTER(i)]
     L.dop_batch_create.argtypes = [i, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     L.dop_batch_apply.argtypes = [vp, i, vp]
     L.dop_batch_autotune.argtypes = [vp, vp, C.POINTER(i)]

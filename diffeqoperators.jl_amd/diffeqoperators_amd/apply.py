@@ -8,11 +8,13 @@ contiguous device tensors (float64, float32 or bfloat16); there is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import torch
 
 from . import _abi
-from .bcs import AffineBC, ComposedMultiDimBC, MultiDimDirectionalBC, PeriodicAtomicBC, _LineBCs
+from .bcs import (AffineBC, ComposedMultiDimBC, MultiDimDirectionalBC, PeriodicAtomicBC, _LineBCs,
+                  all_periodic_lines)
 from .composite import DiffEqOperatorCombination, GhostDerivativeOperator
 from .operators import DerivativeOperator
 
@@ -75,6 +77,24 @@ class Plan:
                                         C.c_void_p(u.data_ptr()), int(bool(accumulate)),
                                         C.c_void_p(stream)), "dop_apply")
 
+    def update(self, desc: _abi.DopDesc, keep, stream=None):
+        """New BC values / operator tables of the same structure, in place
+        (dop_plan_update): the plan's device memory is reused.  Raises DopError (EINVAL) when
+        the structure differs."""
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _abi.check(_abi.lib().dop_plan_update(self.handle, C.byref(desc), C.c_void_p(stream)),
+                   "dop_plan_update")
+        self._keep = keep
+        self.desc = desc
+
+    @property
+    def variant(self):
+        """(candidate launched, tuned?) -- dop_plan_variant."""
+        t = C.c_int()
+        v = _abi.lib().dop_plan_variant(self.handle, C.byref(t))
+        return v, bool(t.value)
+
     def autotune(self, stream=None):
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
@@ -120,6 +140,9 @@ def make_desc(terms, jsize, storage, *, input_kind, semantics, bcs=None, padded=
             flat = bc.flat()
             arr = (_abi.DopAffineBC * len(flat))()
             for i, b in enumerate(flat):
+                if isinstance(b, PeriodicAtomicBC):
+                    arr[i].present, arr[i].kind = 1, _abi.BC_PERIODIC
+                    continue
                 if len(b.a_l) > _abi.MAX_BC_TAPS or len(b.a_r) > _abi.MAX_BC_TAPS:
                     raise ValueError("boundary condition order too high")
                 e = arr[i]
@@ -173,6 +196,8 @@ def _bcs_of(Q, ndim, jsize):
     if isinstance(Q, MultiDimDirectionalBC):
         _check_dir_size(Q, jsize, "Size of the BCs array in the MultiDimBC is incorrect")
         if Q.lines is not None:
+            if all_periodic_lines(Q.lines):
+                return {Q.D - 1: _PeriodicPad(_abi.BC_PERIODIC_ND)}, _abi.SEM_LINES
             return {Q.D - 1: _LineBCs(Q.lines)}, _abi.SEM_LINES
         bc = Q.bc
         if isinstance(bc, PeriodicAtomicBC):
@@ -181,11 +206,12 @@ def _bcs_of(Q, ndim, jsize):
     if isinstance(Q, ComposedMultiDimBC):
         # all atoms periodic: the N-D periodic method (:244-252); otherwise every line is
         # padded by its own atom (slice_rmul), i.e. the line wrap for periodic atoms
-        all_periodic = all(isinstance(b.bc, PeriodicAtomicBC) for b in Q.bcs)
+        all_periodic = all(all_periodic_lines(b.lines) if b.lines is not None
+                           else isinstance(b.bc, PeriodicAtomicBC) for b in Q.bcs)
         out = {}
         for b in Q.bcs:
             _check_dir_size(b, jsize, f"Size of the BCs array for dimension {b.D} is incorrect")
-            if b.lines is not None:
+            if b.lines is not None and not all_periodic:
                 out[b.D - 1] = _LineBCs(b.lines)
                 continue
             bc = b.bc
@@ -238,21 +264,102 @@ def _plan_cache(obj):
     return c
 
 
-def ghost_plans(owner, terms, u: torch.Tensor, accum=_abi.ACC_AUTO):
-    """Plans evaluating sum_t L_t*Q_t*u (cached on owner)."""
+def _op_structure(L):
+    return (L.N, int(L.wind), L.derivative_order, L.approximation_order, L.len,
+            L.stencil_length, L.boundary_stencil_length, L.boundary_point_count, L.offside,
+            int(L.uniform_grid), L.coeff_kind)
+
+
+def _bc_structure(bc):
+    if isinstance(bc, _PeriodicPad):
+        return bc.key()
+    if isinstance(bc, _LineBCs):
+        return ("lines", hash(tuple(("p",) if isinstance(b, PeriodicAtomicBC) else
+                                    (len(b.a_l), len(b.a_r)) for b in bc.flat())))
+    return ("affine", len(bc.a_l), len(bc.a_r))
+
+
+class _Bound:
+    """The plans of one owner (an L*Q object) and the descriptors of its values."""
+
+    def __init__(self, skey, plans, descs):
+        self.skey, self.plans, self.descs = skey, plans, descs
+
+
+# Plans shared by owners of the same structure (same operators' kinds and sizes, same BC kinds
+# and tap counts): a method-of-lines RHS that builds a new BC at every call (heat_equation.jl:
+# 38-43, KdV.jl:26-30) reuses one plan and only replaces its values (dop_plan_update) instead
+# of creating a plan per call.  Bounded: the least recently used structures are dropped.
+_SHARED: "collections.OrderedDict" = None
+_SHARED_MAX = 64
+
+
+def _shared():
+    global _SHARED
+    if _SHARED is None:
+        import collections
+        _SHARED = collections.OrderedDict()
+    return _SHARED
+
+
+def ghost_plans(owner, terms, u: torch.Tensor, accum=_abi.ACC_AUTO, shared=False):
+    """Plans evaluating sum_t L_t*Q_t*u, cached on owner.  shared=True (the implicit `A*u` /
+    `mul_` paths, which use the plans at once): plans of the same structure are shared between
+    owners and rebound to an owner's values on use.  shared=False (plans handed to the caller,
+    who may keep them, e.g. in a batch or a server): the owner's own plans."""
     jsize = julia_size(u)
     storage = storage_of(u)
-    key = ("ghost", jsize, storage, accum)
+    key = ("ghost", jsize, storage, accum, u.device.index)
     cache = _plan_cache(owner)
-    if key not in cache:
-        with torch.cuda.device(u.device):
-            plans = []
-            for ops, bcs, _, sem in _ghost_groups(terms, len(jsize), jsize):
-                d, keep = make_desc(ops, jsize, storage, input_kind=_abi.INPUT_GHOST,
-                                    semantics=sem, bcs=bcs, accum=accum)
-                plans.append(Plan(d, keep + [bcs]))
-        cache[key] = plans
-    return cache[key]
+    if not shared:
+        own = cache.get(key + ("own",))
+        if own is None:
+            with torch.cuda.device(u.device):
+                own = []
+                for ops, bcs, _, sem in _ghost_groups(terms, len(jsize), jsize):
+                    d, keep = make_desc(ops, jsize, storage, input_kind=_abi.INPUT_GHOST,
+                                        semantics=sem, bcs=bcs, accum=accum)
+                    own.append(Plan(d, keep + [bcs]))
+            cache[key + ("own",)] = own
+        return own
+    bound = cache.get(key)
+    if bound is None:
+        descs, groups = [], _ghost_groups(terms, len(jsize), jsize)
+        for ops, bcs, _, sem in groups:
+            d, keep = make_desc(ops, jsize, storage, input_kind=_abi.INPUT_GHOST,
+                                semantics=sem, bcs=bcs, accum=accum)
+            descs.append((d, keep + [bcs]))
+        # plans are created with the kernel selection the environment asks for
+        env = (os.environ.get("DOP_FORCE_GENERIC"), os.environ.get("DOP_TILE_VARIANT"))
+        skey = key + env + (tuple((sem, tuple(_op_structure(L) for L in ops),
+                             tuple(sorted((ax, _bc_structure(bc)) for ax, bc in bcs.items())))
+                            for ops, bcs, _, sem in groups),)
+        bound = cache[key] = _Bound(skey, None, descs)
+    shared = _shared()
+    plans = shared.get(bound.skey)
+    with torch.cuda.device(u.device):
+        if plans is None:
+            plans = [Plan(d, keep) for d, keep in bound.descs]
+            for p in plans:
+                p._owner = bound
+            shared[bound.skey] = plans
+            while len(shared) > _SHARED_MAX:
+                shared.popitem(last=False)
+        else:
+            shared.move_to_end(bound.skey)
+            for i, p in enumerate(plans):
+                if p._owner is bound:
+                    continue
+                d, keep = bound.descs[i]
+                try:
+                    p.update(d, keep)
+                except _abi.DopError:
+                    # same kinds and sizes but a different row structure (e.g. coefficients
+                    # that are no longer uniform): a plan of its own
+                    p = plans[i] = Plan(d, keep)
+                p._owner = bound
+    bound.plans = plans
+    return plans
 
 
 def _ghost_terms(A):
@@ -272,7 +379,7 @@ def _apply_plans(plans, du, u, accumulate):
 def ghost_times(G, u):
     _check_device_tensor(u, "u")
     out = torch.empty_like(u)
-    _apply_plans(ghost_plans(G, _ghost_terms(G), u), out, u, False)
+    _apply_plans(ghost_plans(G, _ghost_terms(G), u, shared=True), out, u, False)
     return out
 
 
@@ -322,7 +429,7 @@ def combination_times(A: DiffEqOperatorCombination, x):
     terms = _ghost_terms(A)
     if terms is not None:
         out = torch.empty_like(x)
-        _apply_plans(ghost_plans(A, terms, x), out, x, False)
+        _apply_plans(ghost_plans(A, terms, x, shared=True), out, x, False)
         return out
     if all(isinstance(op, DerivativeOperator) for op in A.ops):
         shapes = set()
@@ -355,7 +462,7 @@ def mul_(du, A, u, overwrite=True, accum=_abi.ACC_AUTO):
     if terms is not None:
         if tuple(du.shape) != tuple(u.shape):
             raise AssertionError("du must have the unpadded size of u")
-        _apply_plans(ghost_plans(A, terms, u, accum), du, u, accumulate)
+        _apply_plans(ghost_plans(A, terms, u, accum, shared=True), du, u, accumulate)
         return du
     if isinstance(A, DerivativeOperator):
         # derivative_operator_functions.jl:18-69

@@ -19,13 +19,33 @@ def _single_plan(A, u):
     return plans[0]
 
 
+def _check_buffer(t, plan, n_elems, name, i):
+    """A device buffer the library reads or writes for problem i: a contiguous GPU tensor of
+    the plan's storage type and element count (the C side trusts the pointer)."""
+    from .apply import _check_device_tensor, storage_of
+    _check_device_tensor(t, f"{name}[{i}]")
+    if storage_of(t) != plan.storage:
+        raise TypeError(f"{name}[{i}] has dtype {t.dtype}, the plan's storage type differs")
+    if t.numel() != n_elems:
+        raise ValueError(f"{name}[{i}] has {t.numel()} elements, the plan needs {n_elems}")
+
+
 class Batch:
     """du_i = A_i * u_i for all i in one launch (dop_batch_create / dop_batch_apply)."""
 
     def __init__(self, ops, us, dus):
         if not (len(ops) == len(us) == len(dus)) or not ops:
             raise ValueError("need equal, non-empty lists of operators, inputs and outputs")
+        from .apply import _check_device_tensor
+        for i, (u, du) in enumerate(zip(us, dus)):
+            _check_device_tensor(u, f"us[{i}]")
+            _check_device_tensor(du, f"dus[{i}]")
         self.plans = [_single_plan(A, u) for A, u in zip(ops, us)]
+        for i, (p, u, du) in enumerate(zip(self.plans, us, dus)):
+            _check_buffer(u, p, p.in_elems, "us", i)
+            _check_buffer(du, p, p.out_elems, "dus", i)
+            if du.data_ptr() == u.data_ptr():
+                raise ValueError(f"dus[{i}] must not alias us[{i}]")
         self.us, self.dus = list(us), list(dus)
         n = len(ops)
         P = (C.c_void_p * n)(*[p.handle.value for p in self.plans])
@@ -68,7 +88,12 @@ class RK4:
         n = len(ops)
         if not (n == len(us) == len(scale) == len(dt)) or n == 0:
             raise ValueError("need equal, non-empty lists")
+        from .apply import _check_device_tensor
+        for i, u in enumerate(us):
+            _check_device_tensor(u, f"us[{i}]")
         self.plans = [_single_plan(A, u) for A, u in zip(ops, us)]
+        for i, (p, u) in enumerate(zip(self.plans, us)):
+            _check_buffer(u, p, p.in_elems, "us", i)
         self.us = list(us)
         self._scale = np.ascontiguousarray(scale, dtype=np.float64)
         self._dt = np.ascontiguousarray(dt, dtype=np.float64)

@@ -181,13 +181,15 @@ class BoundaryPadded:
             if isinstance(bc, _PeriodicPad):
                 gl, gr = (x[..., n - 1], x[..., 0]) if bc.kind == 1 else (x[..., 0], x[..., n - 1])
             elif isinstance(bc, _LineBCs):
-                al, bl, ar, br = (torch.as_tensor(t, dtype=torch.float64, device=u.device)
-                                  for t in bc.tables())
+                al, bl, ar, br, per = (torch.as_tensor(t, device=u.device) for t in bc.tables())
+                al, bl, ar, br = (t.to(torch.float64) for t in (al, bl, ar, br))
                 xl = x.reshape(-1, n)   # C order over the other axes = Julia's column-major lines
                 k = al.shape[1]
-                gl = ((xl[:, :k] * al).sum(-1) + bl).reshape(x.shape[:-1])
+                gl = (xl[:, :k] * al).sum(-1) + bl
                 k = ar.shape[1]
-                gr = ((xl[:, n - k:] * ar).sum(-1) + br).reshape(x.shape[:-1])
+                gr = (xl[:, n - k:] * ar).sum(-1) + br
+                gl = torch.where(per, xl[:, n - 1], gl).reshape(x.shape[:-1])
+                gr = torch.where(per, xl[:, 0], gr).reshape(x.shape[:-1])
             else:
                 al = torch.as_tensor(bc.a_l, dtype=torch.float64, device=u.device)
                 ar = torch.as_tensor(bc.a_r, dtype=torch.float64, device=u.device)
@@ -204,7 +206,9 @@ class BoundaryPadded:
 
 
 class _LineBCs:
-    """One affine BC per line along an axis (lines in Julia column-major order)."""
+    """One atomic BC per line along an axis (lines in Julia column-major order): affine, or
+    PeriodicBC, whose line wraps (slice_rmul pads line I with BC[I] * u[I, :],
+    multi_dim_bc_operators.jl:29-39; bc_operators.jl:192)."""
 
     def __init__(self, lines):
         self.lines = lines
@@ -216,17 +220,31 @@ class _LineBCs:
         return list(self.lines.flatten(order="F"))
 
     def tables(self):
-        """(a_l, b_l, a_r, b_r) per line, taps right-aligned for a_r and zero-padded."""
+        """(a_l, b_l, a_r, b_r, periodic) per line, taps right-aligned for a_r and
+        zero-padded; periodic lines carry no taps."""
         bcs = self.flat()
-        kl = max(len(b.a_l) for b in bcs)
-        kr = max(len(b.a_r) for b in bcs)
+        per = np.array([isinstance(b, PeriodicAtomicBC) for b in bcs])
+        aff = [b for b in bcs if not isinstance(b, PeriodicAtomicBC)]
+        kl = max([len(b.a_l) for b in aff], default=0)
+        kr = max([len(b.a_r) for b in aff], default=0)
         al = np.zeros((len(bcs), kl))
         ar = np.zeros((len(bcs), kr))
+        bl = np.zeros(len(bcs))
+        br = np.zeros(len(bcs))
         for i, b in enumerate(bcs):
+            if per[i]:
+                continue
             al[i, : len(b.a_l)] = b.a_l
             if len(b.a_r):
                 ar[i, kr - len(b.a_r):] = b.a_r
-        return al, np.array([b.b_l for b in bcs]), ar, np.array([b.b_r for b in bcs])
+            bl[i], br[i] = b.b_l, b.b_r
+        return al, bl, ar, br, per
+
+
+def all_periodic_lines(lines) -> bool:
+    """A BC array whose entries are all PeriodicBC (Julia eltype PeriodicBC{T}), which takes the
+    N-D periodic padding (multi_dim_bc_operators.jl:221-228) instead of slice_rmul."""
+    return all(isinstance(b, PeriodicAtomicBC) for b in np.asarray(lines, dtype=object).flat)
 
 
 def _bc_times(Q, u):
@@ -320,9 +338,9 @@ def _multidim_dim(dim, bc, s=None):
         if lines.ndim < 1 or lines.ndim > 2:
             raise ValueError("the BC array must have 1 or 2 dimensions (2-D or 3-D arrays)")
         for b in lines.flat:
-            if not isinstance(b, AffineBC):
-                raise TypeError("per-line boundary conditions must be affine (Robin / Dirichlet / "
-                                "Neumann / GeneralBC)")
+            if not isinstance(b, (AffineBC, PeriodicAtomicBC)):
+                raise TypeError("per-line boundary conditions must be atomic (Robin / Dirichlet / "
+                                "Neumann / GeneralBC / PeriodicBC)")
         return MultiDimDirectionalBC(lines.flat[0], dim, None, lines=lines)
     return MultiDimDirectionalBC(bc, dim, s)
 

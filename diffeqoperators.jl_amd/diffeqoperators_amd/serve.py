@@ -29,8 +29,15 @@ class Server:
                release_s: float) -> int:
         """u0: pinned host tensor with the plan's size and dtype; returns the request id.
         The final state lands in self.results[id] (pinned host tensor)."""
-        if u0.device.type != "cpu" or not u0.is_pinned():
+        from .apply import storage_of
+        if not isinstance(u0, torch.Tensor) or u0.device.type != "cpu" or not u0.is_pinned():
             raise ValueError("u0 must be a pinned host tensor")
+        if not u0.is_contiguous():
+            raise ValueError("u0 must be contiguous")
+        if storage_of(u0) != plan.storage:
+            raise TypeError(f"u0 has dtype {u0.dtype}, the plan's storage type differs")
+        if u0.numel() != plan.in_elems:
+            raise ValueError(f"u0 has {u0.numel()} elements, the plan needs {plan.in_elems}")
         out = torch.empty_like(u0).pin_memory()
         rid = C.c_int64()
         _abi.check(_abi.lib().dop_server_submit(

@@ -19,8 +19,9 @@
  *     caller owns them and the stream.  du must not alias u.
  *   - Every function returns DOP_OK (0) or a negative DOP_E* code; dop_last_error()
  *     returns a thread-local message for the last failure.  No exception crosses the ABI.
- *   - Plans are immutable after creation: concurrent dop_apply calls on one plan from
- *     several threads/streams are safe.
+ *   - dop_apply only reads the plan: concurrent dop_apply calls on one plan from several
+ *     threads/streams are safe.  dop_plan_update and dop_plan_autotune change the plan and
+ *     must not run while another thread uses it.
  */
 #ifndef DOP_H
 #define DOP_H
@@ -139,10 +140,13 @@ typedef struct dop_desc {
     int32_t transverse_view;
     const dop_operator* ops;          /* [n_ops], applied as sum_t */
     dop_affine_bc bc[3];              /* GHOST input: per axis */
-    /* GHOST input, optional per axis: one affine BC per line along that axis (MultiDimBC{dim}
+    /* GHOST input, optional per axis: one atomic BC per line along that axis (MultiDimBC{dim}
      * of an Array of atomic BCs, multi_dim_bc_operators.jl:93-95; line l of axis a is the
      * column-major index over the other axes), or NULL for the single BC bc[a] (whose
-     * present flag must still be set; its taps are then unused). */
+     * present flag must still be set, with kind DOP_BC_AFFINE; its taps are then unused).
+     * An entry is DOP_BC_AFFINE (its own taps) or DOP_BC_PERIODIC: slice_rmul pads line l
+     * with BC[l] * u[line] (:29-39), which for a PeriodicBC is the wrap u[end], u[1]
+     * (bc_operators.jl:192). */
     const dop_affine_bc* line_bc[3];
 } dop_desc;
 
@@ -157,6 +161,21 @@ const char* dop_last_error(void);
  * 178-192): the device-side form of what those lazy objects hold. */
 int dop_plan_create(const dop_desc* desc, dop_plan** out);
 void dop_plan_destroy(dop_plan* plan);
+
+/* New values for an existing plan, in place: the BCs' taps and constants (also per line) and
+ * the operators' tables and coefficients of `desc`, which must describe the same structure as
+ * the descriptor the plan was created from (ndim, sizes, storage, accumulation, input kind,
+ * semantics, the operators' axes, kinds, orders, lengths and grid kind, the BCs' kinds and tap
+ * counts; DOP_EINVAL otherwise, the plan unchanged).  Replaces what a method-of-lines RHS does
+ * when it rebuilds its BC (or calls update_coefficients!) at every call before `A*u`
+ * (heat_equation.jl:38-43, KdV.jl:26-30, DiffEqOperators.jl:67-75): the plan's device memory
+ * is reused, nothing is allocated.  When only the BC values change, the new values travel as a
+ * kernel argument on `stream` (no host synchronization; capturable); changed tables are copied
+ * on `stream` and the call waits for those copies.  Applies enqueued on `stream` before the
+ * update see the old values, later ones the new; the caller orders applies on other streams.
+ * Batches, RK4 integrators and servers created from the plan read its values on the device,
+ * so their later launches see the update. */
+int dop_plan_update(dop_plan* plan, const dop_desc* desc, void* stream);
 
 /* Number of bytes of u and du the plan reads/writes per dop_apply (storage elements). */
 int64_t dop_plan_input_elems(const dop_plan* plan);
@@ -176,7 +195,9 @@ int dop_apply(const dop_plan* plan, void* du, const void* u, int accumulate, voi
 /* ---- batching (no counterpart in the reference: it applies one operator per call) ----
  * A batch binds n independent plans (any mix of ndim, sizes, operators, BCs; one storage
  * type) to their du / u device pointers once; dop_batch_apply then evaluates all of them
- * in ONE persistent-kernel launch. */
+ * from one device work list: its items are sorted into groups of one kind (2-D tiles, 1-D
+ * tiles, generic points), window width and accumulator, and each group is one launch of a
+ * kernel specialised for it (a handful of launches for any number of problems). */
 typedef struct dop_batch dop_batch;
 int dop_batch_create(int n, const dop_plan* const* plans, void* const* du,
                      const void* const* u, dop_batch** out);
@@ -189,7 +210,8 @@ void dop_batch_destroy(dop_batch* batch);
 /* ---- explicit time stepping (no counterpart in the reference; its RHS is called from
  * OrdinaryDiffEq, heat_equation.jl:30-33) ----
  * A batch of method-of-lines problems du/dt = scale_i * (sum_t L_t Q u) advanced with
- * classical RK4, every stage of the whole batch one persistent launch, the four stages of
+ * classical RK4, every stage of the whole batch one grouped launch set (as dop_batch_apply; the
+ * groups of a stage run on forked streams inside the graph), the four stages of
  * a step captured once in a hipGraph and replayed.  Storage f32 or f64.  Creation tunes
  * the tile shape (DOP_RK4_AUTOTUNE=0 skips it) by timing stage-0 launches, which read u and
  * write only the internal stage buffers. */
@@ -207,7 +229,7 @@ void dop_rk4_destroy(dop_rk4* rk);
  * replays the releases in time, admits released requests in FIFO order while their state
  * (u and three RK4 stage buffers, carved from one device arena of `hbm_budget_bytes`) fits
  * and fewer than `max_active` are resident, advances every resident request by one RK4
- * step per iteration (each stage one persistent launch over all of them), and copies each
+ * step per iteration (each stage one grouped launch set over all of them), and copies each
  * finished state back asynchronously; requests that arrive while others run join at the
  * next iteration.  Storage f32 or f64 (one type per server). */
 typedef struct dop_server dop_server;
@@ -246,7 +268,6 @@ int dop_plan_csr_rows(const dop_plan* plan, int64_t* row_ptr, int64_t* nnz, int6
                       int64_t* ku, void* stream);
 int dop_plan_csr_fill(const dop_plan* plan, const int64_t* row_ptr, int64_t* col, double* val,
                       double* b, void* stream);
-
 /* ---- direct solve: A \\ u = A_l \\ (u - A_b) (ghost_derivative_operator.jl:39-58) ----
  * dop_solver_create concretizes a square GHOST plan on the device and factors its band with
  * partial pivoting (LU, the LAPACK gbtf2 algorithm) once; DOP_ESINGULAR on a zero pivot (the
@@ -295,9 +316,17 @@ void dop_tsit5_destroy(dop_tsit5* integ);
 
 /* ---- tile autotuner (no counterpart in the reference) ----
  * Times the candidate tile shapes of the plan's fused kernel on scratch buffers and keeps
- * the fastest in the plan (and in a process-wide cache keyed by dtype, window and
- * extents).  Returns the chosen candidate index via *choice (may be NULL). */
+ * the fastest in the plan and in a process-wide cache keyed by device, storage,
+ * accumulation, window width, grid kind and extents.  Returns the chosen candidate index via
+ * *choice (may be NULL).  Plans created for a key already in the cache start with its
+ * variant; a plan with a tiled path of at least 2^20 points that is still untuned tunes on its
+ * first dop_apply with accumulate = 0 (timing the candidates on that call's own du and u, then
+ * applying), unless the stream is being captured or DOP_AUTOTUNE=0 is set.  Every candidate
+ * computes the same values bit for bit: the choice changes only the time. */
 int dop_plan_autotune(dop_plan* plan, void* stream, int* choice);
+/* The candidate the plan launches (the autotuner's choice), and whether it was tuned (1) or
+ * is the default (0). */
+int dop_plan_variant(const dop_plan* plan, int* tuned);
 
 #ifdef __cplusplus
 }

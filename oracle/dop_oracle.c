@@ -722,9 +722,11 @@ int orc_ghost_apply(int nops, const orc_op* ops, int ndim, const long long* size
         if (q->lines) {
             const long long nl_lines = n / size[ops[t].axis];
             for (long long l = 0; l < nl_lines; ++l) {
+                /* a line's own atomic BC: affine, or PeriodicBC (the line wraps) */
                 const orc_bc* e = &q->lines[l];
-                if (e->periodic || e->nl < 0 || e->nr < 0 || e->nl > size[ops[t].axis] ||
-                    e->nr > size[ops[t].axis])
+                if (e->periodic != 0 && e->periodic != 1) return ORC_EINVAL;
+                if (!e->periodic && (e->nl < 0 || e->nr < 0 || e->nl > size[ops[t].axis] ||
+                                     e->nr > size[ops[t].axis]))
                     return ORC_EINVAL;
             }
         }

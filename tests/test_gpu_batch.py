@@ -137,3 +137,33 @@ def test_rk4_graph_replays_are_consistent():
     torch.cuda.synchronize()
     for x, y in zip(us_a, us_b):
         assert torch.equal(x, y)
+
+
+def test_batch_and_rk4_reject_mismatched_buffers():
+    # a buffer of another dtype or length than the plan's would be read or written out of
+    # bounds on the device: refused before the batch is created
+    n = 100
+    h = 1.0 / (n + 1)
+    A = D.CenteredDifference(2, 2, h, n) * D.Dirichlet0BC(float)
+    u = torch.zeros(n, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        D.Batch([A], [u], [torch.zeros(n - 1, dtype=torch.float64, device="cuda")])
+    with pytest.raises(TypeError):
+        D.Batch([A], [u], [torch.zeros(n, dtype=torch.float32, device="cuda")])
+    with pytest.raises(ValueError):
+        D.Batch([A], [u], [u])
+
+
+def test_server_submit_rejects_mismatched_u0():
+    n = 64
+    h = 1.0 / (n + 1)
+    A = (D.CenteredDifference[1](2, 4, h, n) + D.CenteredDifference[2](2, 4, h, n)) * \
+        D.compose(*D.Dirichlet0BC(float, (n, n)))
+    plan = D.plan_for_shape(A, (n, n), torch.float32)
+    srv = D.Server(64 << 20)
+    with pytest.raises(ValueError):
+        srv.submit(plan, torch.zeros(n * n - 1, dtype=torch.float32).pin_memory(), 1.0, 1e-5, 2, 0.0)
+    with pytest.raises(TypeError):
+        srv.submit(plan, torch.zeros(n * n, dtype=torch.float64).pin_memory(), 1.0, 1e-5, 2, 0.0)
+    with pytest.raises(ValueError):
+        srv.submit(plan, torch.zeros(n * n, dtype=torch.float32), 1.0, 1e-5, 2, 0.0)

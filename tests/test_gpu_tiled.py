@@ -71,7 +71,7 @@ def test_tiled_2d_bitwise(shape, kh, dtype, uniform):
 
 
 CASES_3D = [((32, 32, 32), 1), ((40, 48, 64), 2), ((33, 29, 70), 3), ((64, 64, 256), 3),
-            ((17, 100, 36), 4), ((96, 96, 96), 3)]
+            ((17, 100, 36), 4), ((96, 96, 96), 3), ((12, 40, 600), 3), ((19, 23, 532), 2)]
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
@@ -123,11 +123,13 @@ def n_variants(shape, dtype):
     vx = vx if n0 % vx == 0 else 1
     if len(shape) == 2:
         return 32 if vx > 1 and n0 % (2 * vx) == 0 else 16
-    return 40 if vx > 1 else 20
+    # overlapping segments (40 / 20) and edge-loaded segments (32 / 16)
+    return 72 if vx > 1 else 36
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
-@pytest.mark.parametrize("shape,kh", [((300, 260), 2), ((40, 72, 100), 3), ((33, 96), 5)])
+@pytest.mark.parametrize("shape,kh", [((300, 260), 2), ((40, 72, 100), 3), ((33, 96), 5),
+                                      ((21, 45, 532), 3), ((14, 30, 266), 4)])
 def test_every_tile_variant_bitwise(shape, kh, dtype):
     nvar = n_variants(shape, dtype)
     A, u = build(shape, kh, dtype, seed=5, uniform=False, upwind=(kh == 3))

@@ -13,7 +13,8 @@ import torch
 import pyoracle as po
 import diffeqoperators_amd as D
 from diffeqoperators_amd import _abi
-from diffeqoperators_amd.apply import _bcs_of, make_desc
+from diffeqoperators_amd.apply import _PeriodicPad, _bcs_of, make_desc
+from diffeqoperators_amd.bcs import PeriodicAtomicBC
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HC_PATH = os.path.join(ROOT, "tests", "hostcheck", "_build", "libdop_hostcheck.so")
@@ -42,14 +43,17 @@ def random_bc(rng, dx):
     return D.GeneralBC([rng.uniform(0.5, 1.0), 1.0, 0.3], [rng.uniform(0.5, 1.0), 1.0, -0.2], dx, 2)
 
 
-def bc_array(rng, shape, dx):
+def bc_array(rng, shape, dx, periodic=0.0):
+    """Random atomic BCs, a fraction `periodic` of them PeriodicBC."""
     a = np.empty(shape, dtype=object)
     for i in np.ndindex(*shape):
-        a[i] = random_bc(rng, dx)
+        a[i] = D.PeriodicBC(float) if rng.random() < periodic else random_bc(rng, dx)
     return a
 
 
 def obc(b):
+    if isinstance(b, PeriodicAtomicBC):
+        return po.periodic()   # the line wraps (bc_operators.jl:192)
     return po.BC(b.a_l, b.b_l, b.a_r, b.b_r)
 
 
@@ -83,8 +87,42 @@ def test_oracle_per_line_equals_line_loop(rng):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("shape", [(12, 17), (5, 6, 7)])
-def test_hostcheck_per_line_matches_oracle(hc, rng, shape):
+def test_oracle_per_line_periodic_pattern(rng):
+    # multi_dim_bc_test.jl:7-35: half the lines Robin, half PeriodicBC; each padded line equals
+    # its own atomic BC applied to that line (Ax[:, j] == Array(BCx[j] * A[:, j]))
+    n, m = 8, 15
+    A = rng.random((m, n))                                  # Julia size (n, m)
+    q1 = D.RobinBC((1.0, 2.0, 3.0), (0.0, -1.0, 2.0), 0.1, 4)
+    q2 = D.PeriodicBC(float)
+    BCx = [q1] * (m // 2) + [q2] * (m - m // 2)
+    L = D.CenteredDifference[1](2, 2, 0.1, n)
+    got = po.ghost_apply([oop(L)], [po.BC.per_line([obc(b) for b in BCx]), po.dirichlet0()], A,
+                         bpv_1d=False)
+    for j in range(m):
+        line = A[j, :]
+        gl, gr = (line[-1], line[0]) if j >= m // 2 else obc(q1).pad(line)
+        padded = np.concatenate([[gl], line, [gr]])
+        ref = oop(L).mul_padded(padded)
+        # the ghost of a Robin line is summed in another order here (numpy's dot)
+        assert np.abs(got[j, :] - ref).max() <= 1e-13 * np.abs(ref).max(), j
+
+
+def test_all_periodic_lines_take_the_nd_padding():
+    # an array whose eltype is PeriodicBC takes the N-D method (multi_dim_bc_operators.jl:221-228)
+    arr = np.empty((6,), dtype=object)
+    for i in range(6):
+        arr[i] = D.PeriodicBC(float)
+    bcs, _ = _bcs_of(D.MultiDimBC[1](arr), 2, (5, 6))
+    assert isinstance(bcs[0], _PeriodicPad) and bcs[0].kind == _abi.BC_PERIODIC_ND
+    # mixed: one BC per line
+    arr[2] = D.Dirichlet0BC(float)
+    bcs, _ = _bcs_of(D.MultiDimBC[1](arr), 2, (5, 6))
+    assert not isinstance(bcs[0], _PeriodicPad)
+
+
+@pytest.mark.parametrize("shape,periodic", [((12, 17), 0.0), ((5, 6, 7), 0.0), ((12, 17), 0.5),
+                                             ((5, 6, 7), 0.4)])
+def test_hostcheck_per_line_matches_oracle(hc, rng, shape, periodic):
     jsize = tuple(reversed(shape))
     nd = len(shape)
     u = rng.standard_normal(shape)
@@ -95,7 +133,7 @@ def test_hostcheck_per_line_matches_oracle(hc, rng, shape):
         ops += [D.CenteredDifference[N](2, 4, dx, n), D.UpwindDifference[N](1, 2, dx, n, rng.standard_normal(n))]
         oops += [oop(ops[-2]), oop(ops[-1])]
         other = tuple(jsize[k] for k in range(nd) if k != N - 1)
-        arr = bc_array(rng, other, 0.05)
+        arr = bc_array(rng, other, 0.05, periodic)
         Q = D.MultiDimBC[N](arr)
         bcs.update(_bcs_of(Q, nd, jsize)[0])
         obcs.append(po.BC.per_line([obc(b) for b in arr.flatten(order="F")]))
@@ -141,6 +179,19 @@ def test_materialize_per_line(rng):
         gr = np.dot(b.a_r, un[i2, n1 - len(b.a_r):]) + b.b_r
         assert P[i2 + 1, 0] == pytest.approx(gl, rel=1e-14, abs=1e-14)
         assert P[i2 + 1, -1] == pytest.approx(gr, rel=1e-14, abs=1e-14)
+
+
+def test_materialize_per_line_periodic(rng):
+    n1, n2 = 7, 6
+    u = torch.from_numpy(rng.standard_normal((n2, n1)))
+    arr = bc_array(rng, (n2,), 0.1, periodic=0.5)
+    arr[0], arr[1] = D.PeriodicBC(float), D.Dirichlet0BC(float)
+    P = (D.MultiDimBC[1](arr) * u).materialize().numpy()
+    un = u.numpy()
+    for i2 in range(n2):
+        gl, gr = obc(arr[i2]).pad(un[i2])
+        assert P[i2, 0] == pytest.approx(gl, rel=1e-14, abs=1e-14)
+        assert P[i2, -1] == pytest.approx(gr, rel=1e-14, abs=1e-14)
 
 
 def test_per_line_size_mismatch_raises():

@@ -8,6 +8,7 @@ import re
 
 import numpy as np
 import pytest
+import torch
 
 import pyoracle as po
 import diffeqoperators_amd as D
@@ -408,3 +409,18 @@ def test_hc_general_bc(hc, rng, al, ar, dx, order):
     got = hc_apply(hc, ops, U, bcs={0: bc, 1: bc})
     ref = po.ghost_apply([_oracle_op(L2, L2.N - 1) for L2 in ops], [_obc(bc), _obc(bc)], U)
     assert rel(got, ref) < 1e-13
+
+
+# ---------------------------------------------------------------------------------------
+# buffers handed to the batched and RK4 entry points are checked before anything is created
+
+def test_batch_and_rk4_reject_host_tensors():
+    h = 1.0 / 33
+    A = D.CenteredDifference(2, 2, h, 32) * D.Dirichlet0BC(float)
+    u = torch.zeros(32, dtype=torch.float64)
+    with pytest.raises(TypeError):
+        D.Batch([A], [u], [torch.zeros_like(u)])
+    with pytest.raises(TypeError):
+        D.RK4([A], [u], [1.0], [1e-4])
+    with pytest.raises(ValueError):
+        D.Batch([A], [u], [])
