@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../include/dop.h"
@@ -569,6 +570,88 @@ int efail(int code, const std::string& m) { return dop::set_error(code, m); }
     } while (0)
 
 constexpr int kMaxRowEntries = 128;
+
+// Checks shared by both row-pointer exports.
+int csr_rows_check(const dop_plan* plan, const int64_t* row_ptr, const int64_t* nnz)
+{
+    if (!plan || !row_ptr || !nnz) return efail(DOP_EINVAL, "null plan, row_ptr or nnz");
+    if (dop::row_entry_bound(plan->d.host) > kMaxRowEntries)
+        return efail(DOP_EUNSUPPORTED, "rows of this plan have more than 128 entries");
+    return DOP_OK;
+}
+
+// Row counts, their scan into row_ptr[1..n] and the bandwidths.  row_ptr[0] and band[0..1]
+// must already be zero on the stream; part holds one partial sum per scan tile.  Synchronizes
+// the stream.
+int csr_rows_into(const dop_plan* plan, int64_t* row_ptr, int64_t* nnz, int64_t* kl, int64_t* ku,
+                  int64_t* part, unsigned long long* band, hipStream_t s)
+{
+    const int64_t n = plan->d.host.total;
+    const int64_t ntiles = (n + dop::kScanTile - 1) / dop::kScanTile;
+    int rc = DOP_OK;
+    unsigned long long hb[2] = {0, 0};
+    EXP_HIP(dop::csr_pass(plan, false, row_ptr, nullptr, nullptr, nullptr, band, s));
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(dop::k_scan_tiles, dim3((unsigned)ntiles), dim3(dop::kExpBlock), 0, s,
+                           row_ptr + 1, n, part);
+        hipLaunchKernelGGL(dop::k_scan_parts, dim3(1), dim3(dop::kExpBlock), 0, s, part, ntiles);
+        hipLaunchKernelGGL(dop::k_scan_add, dim3((unsigned)ntiles), dim3(dop::kExpBlock), 0, s,
+                           row_ptr + 1, n, part);
+        EXP_HIP(hipGetLastError());
+    }
+    EXP_HIP(hipMemcpyAsync(nnz, row_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    EXP_HIP(hipMemcpyAsync(hb, band, sizeof(hb), hipMemcpyDeviceToHost, s));
+    EXP_HIP(hipStreamSynchronize(s));
+    if (kl) *kl = (int64_t)hb[0];
+    if (ku) *ku = (int64_t)hb[1];
+done:
+    return rc;
+}
+
+// ---- scratch of dop_plan_csr_rows_noalloc ----
+// One static device array holds the bandwidth pair (first 256 bytes) and one partial sum per
+// scan tile, so the export allocates nothing.  Each GPU has its own copy of the array at its
+// own address: the address is taken once per device with hipGetSymbolAddress and only that
+// address is handed to kernels and copies (the array's name is not a device address in host
+// code).  Every call on a GPU shares that GPU's copy: overlapping calls on one GPU race.
+constexpr int64_t kNoAllocTiles = 65536;   // rows <= kNoAllocTiles * kScanTile = 2^28
+constexpr size_t kScratchBandBytes = 256;
+constexpr size_t kScratchBytes = kScratchBandBytes + sizeof(int64_t) * kNoAllocTiles;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+int scratch_base(void** out)
+{
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return efail(DOP_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return efail(DOP_EINVAL, "device index beyond 64");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t g = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (g != hipSuccess || !p)
+            return efail(DOP_EHIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                       hipGetErrorString(g));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return DOP_OK;
+}
+
+__global__ void k_clear_csr_start(unsigned long long* __restrict__ band, int64_t* __restrict__ row_ptr)
+{
+    if (threadIdx.x < 2) band[threadIdx.x] = 0;
+    if (threadIdx.x == 0) row_ptr[0] = 0;
+}
+
 }  // namespace
 
 struct dop_solver {
@@ -589,37 +672,45 @@ extern "C" {
 int dop_plan_csr_rows(const dop_plan* plan, int64_t* row_ptr, int64_t* nnz, int64_t* kl,
                       int64_t* ku, void* stream)
 {
-    if (!plan || !row_ptr || !nnz) return efail(DOP_EINVAL, "null plan, row_ptr or nnz");
-    if (dop::row_entry_bound(plan->d.host) > kMaxRowEntries)
-        return efail(DOP_EUNSUPPORTED, "rows of this plan have more than 128 entries");
+    int rc = csr_rows_check(plan, row_ptr, nnz);
+    if (rc != DOP_OK) return rc;
     const hipStream_t s = (hipStream_t)stream;
-    const int64_t n = plan->d.host.total;
-    const int64_t ntiles = (n + dop::kScanTile - 1) / dop::kScanTile;
-    int rc = DOP_OK;
+    const int64_t ntiles = (plan->d.host.total + dop::kScanTile - 1) / dop::kScanTile;
     int64_t* part = nullptr;
     unsigned long long* band = nullptr;
-    unsigned long long hb[2] = {0, 0};
     EXP_HIP(hipMalloc((void**)&part, sizeof(int64_t) * std::max<int64_t>(ntiles, 1)));
-    EXP_HIP(hipMalloc((void**)&band, sizeof(hb)));
-    EXP_HIP(hipMemsetAsync(band, 0, sizeof(hb), s));
+    EXP_HIP(hipMalloc((void**)&band, 2 * sizeof(unsigned long long)));
+    EXP_HIP(hipMemsetAsync(band, 0, 2 * sizeof(unsigned long long), s));
     EXP_HIP(hipMemsetAsync(row_ptr, 0, sizeof(int64_t), s));
-    EXP_HIP(dop::csr_pass(plan, false, row_ptr, nullptr, nullptr, nullptr, band, s));
-    if (ntiles > 0) {
-        hipLaunchKernelGGL(dop::k_scan_tiles, dim3((unsigned)ntiles), dim3(dop::kExpBlock), 0, s,
-                           row_ptr + 1, n, part);
-        hipLaunchKernelGGL(dop::k_scan_parts, dim3(1), dim3(dop::kExpBlock), 0, s, part, ntiles);
-        hipLaunchKernelGGL(dop::k_scan_add, dim3((unsigned)ntiles), dim3(dop::kExpBlock), 0, s,
-                           row_ptr + 1, n, part);
-        EXP_HIP(hipGetLastError());
-    }
-    EXP_HIP(hipMemcpyAsync(nnz, row_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    EXP_HIP(hipMemcpyAsync(hb, band, sizeof(hb), hipMemcpyDeviceToHost, s));
-    EXP_HIP(hipStreamSynchronize(s));
-    if (kl) *kl = (int64_t)hb[0];
-    if (ku) *ku = (int64_t)hb[1];
+    rc = csr_rows_into(plan, row_ptr, nnz, kl, ku, part, band, s);
 done:
     if (part) (void)hipFree(part);
     if (band) (void)hipFree(band);
+    return rc;
+}
+
+int dop_plan_csr_rows_noalloc(const dop_plan* plan, int64_t* row_ptr, int64_t* nnz, int64_t* kl,
+                              int64_t* ku, void* stream)
+{
+    int rc = csr_rows_check(plan, row_ptr, nnz);
+    if (rc != DOP_OK) return rc;
+    const int64_t ntiles = (plan->d.host.total + dop::kScanTile - 1) / dop::kScanTile;
+    if (ntiles > kNoAllocTiles)
+        return efail(DOP_EBUDGET, "dop_plan_csr_rows_noalloc: the plan has " +
+                                      std::to_string(plan->d.host.total) +
+                                      " rows; its static scratch holds the scan of at most " +
+                                      std::to_string(kNoAllocTiles * dop::kScanTile) + " rows");
+    void* base = nullptr;
+    if ((rc = scratch_base(&base)) != DOP_OK) return rc;
+    unsigned long long* band = (unsigned long long*)base;
+    int64_t* part = (int64_t*)((char*)base + kScratchBandBytes);
+    const hipStream_t s = (hipStream_t)stream;
+    // the scratch keeps the previous call's contents: the bandwidth maxima and row_ptr[0]
+    // start at zero (a kernel, not a memset, so that a captured stream records no memset node)
+    hipLaunchKernelGGL(k_clear_csr_start, dim3(1), dim3(64), 0, s, band, row_ptr);
+    EXP_HIP(hipGetLastError());
+    rc = csr_rows_into(plan, row_ptr, nnz, kl, ku, part, band, s);
+done:
     return rc;
 }
 

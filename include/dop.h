@@ -268,6 +268,18 @@ int dop_plan_csr_rows(const dop_plan* plan, int64_t* row_ptr, int64_t* nnz, int6
                       int64_t* ku, void* stream);
 int dop_plan_csr_fill(const dop_plan* plan, const int64_t* row_ptr, int64_t* col, double* val,
                       double* b, void* stream);
+
+/* Step 1 of the export without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same results as dop_plan_csr_rows, but the scan's partial sums and the
+ * bandwidth counters live in a static device array of the library (512 KiB + 256 bytes per
+ * GPU, held until the process ends) instead of buffers allocated per call: at most 2^28 rows
+ * (DOP_EBUDGET beyond).  Every call on a GPU shares that GPU's array, so calls that overlap on
+ * one GPU (several host threads, several streams) race; serialize them.  Synchronizes the
+ * stream, like dop_plan_csr_rows. */
+int dop_plan_csr_rows_noalloc(const dop_plan* plan, int64_t* row_ptr, int64_t* nnz, int64_t* kl,
+                              int64_t* ku, void* stream);
+
 /* ---- direct solve: A \\ u = A_l \\ (u - A_b) (ghost_derivative_operator.jl:39-58) ----
  * dop_solver_create concretizes a square GHOST plan on the device and factors its band with
  * partial pivoting (LU, the LAPACK gbtf2 algorithm) once; DOP_ESINGULAR on a zero pivot (the
