@@ -516,12 +516,22 @@ def bench_served(nreq=200, lam=2000.0, nsteps=100, budget=8 << 30):
     # includes the arrival gaps: a lower bound on the rate while requests are resident)
     tot_bytes = sum(64.0 * n * n * nsteps for _, n, _, _, _ in reqs)
     ach = tot_bytes / makespan / 1e9
+    sizes = np.array([n * n for _, n, _, _, _ in reqs], dtype=np.float64)
+    cuts = np.percentile(sizes, [100 / 3, 200 / 3])
+    terc = np.digitize(sizes, cuts)    # 0: smallest third of the grids, 2: largest
+
+    def by_tercile(l):
+        return {f"tercile{k}": dict(points_max=int(sizes[terc == k].max()),
+                                     p50_ms=round(float(np.percentile(l[terc == k], 50)) * 1e3, 2),
+                                     p99_ms=round(float(np.percentile(l[terc == k], 99)) * 1e3, 2))
+                for k in range(3) if np.any(terc == k)}
+
     served = dict(solves_per_s=round(nreq / makespan, 1), p50_ms=round(np.percentile(lat, 50) * 1e3, 2),
                   p99_ms=round(np.percentile(lat, 99) * 1e3, 2), makespan_s=round(makespan, 3),
                   roofline=dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                                 frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
                                 bytes_total=tot_bytes),
-                  **srv.stats())
+                  by_size=by_tercile(lat), **srv.stats())
     # plain: FIFO, one request at a time
     lat_p, t0 = [], None
     res_plain = []
@@ -547,7 +557,8 @@ def bench_served(nreq=200, lam=2000.0, nsteps=100, budget=8 << 30):
     mk_p = time.perf_counter() - start - reqs[0][4]
     lat_p = np.array(lat_p)
     plain = dict(solves_per_s=round(nreq / mk_p, 1), p50_ms=round(np.percentile(lat_p, 50) * 1e3, 2),
-                 p99_ms=round(np.percentile(lat_p, 99) * 1e3, 2), makespan_s=round(mk_p, 3))
+                 p99_ms=round(np.percentile(lat_p, 99) * 1e3, 2), makespan_s=round(mk_p, 3),
+                 by_size=by_tercile(lat_p))
     # equal accuracy: the two forms integrate the same RK4 (FP32 storage)
     diff = max(float((srv.results[i] - res_plain[i]).abs().max() / res_plain[i].abs().max())
                for i in range(nreq))

@@ -103,6 +103,9 @@ def lib():
     L.dop_server_submit.argtypes = [vp, vp, C.c_double, C.c_double, i, vp, vp, C.c_double,
                                     C.POINTER(C.c_int64)]
     L.dop_server_run.argtypes = [vp]
+    L.dop_server_set_open.argtypes = [vp, i]
+    L.dop_server_now.argtypes = [vp]
+    L.dop_server_now.restype = C.c_double
     L.dop_server_times.argtypes = [vp, C.c_int64, _dp, _dp, _dp]
     L.dop_server_stats.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.dop_server_destroy.argtypes = [vp]

@@ -1,7 +1,9 @@
 """Continuous-batching method-of-lines service (dop_server_*, no counterpart in the
-reference): requests are released on a trace of arrival times, admitted FIFO while their
-device state fits the HBM budget, advanced together one RK4 step per iteration (one
-persistent launch per stage over all resident requests) and copied back when done."""
+reference): requests are released at their arrival times, admitted FIFO while their device
+state fits the HBM budget, advanced one RK4 step per iteration within their size class (each
+class on its own stream, the classes side by side on the GPU) and copied back when done.
+Requests may be submitted from another thread while `run` runs (`open()` keeps it waiting for
+them until `close()`)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -23,12 +25,15 @@ class Server:
                    "dop_server_create")
         self.handle = h
         self._keep = []
-        self.results = []
+        self.results = {}   # request id -> pinned host tensor with the final state
 
     def submit(self, plan, u0: torch.Tensor, scale: float, dt: float, nsteps: int,
-               release_s: float) -> int:
+               release_s: float = None) -> int:
         """u0: pinned host tensor with the plan's size and dtype; returns the request id.
-        The final state lands in self.results[id] (pinned host tensor)."""
+        The final state lands in self.results[id] (pinned host tensor).  release_s: seconds
+        after the run starts (None: now, or the start of the next run)."""
+        if release_s is None:
+            release_s = max(self.now(), 0.0)
         from .apply import storage_of
         if not isinstance(u0, torch.Tensor) or u0.device.type != "cpu" or not u0.is_pinned():
             raise ValueError("u0 must be a pinned host tensor")
@@ -44,12 +49,25 @@ class Server:
             self.handle, plan.handle, float(scale), float(dt), int(nsteps),
             C.c_void_p(u0.data_ptr()), C.c_void_p(out.data_ptr()), float(release_s),
             C.byref(rid)), "dop_server_submit")
-        self._keep.append((plan, u0))
-        self.results.append(out)
+        self._keep.append((plan, u0, out))
+        self.results[rid.value] = out
         return rid.value
 
     def run(self):
+        """Serves until every submitted request is done and the server is not open (the call
+        releases the GIL: other threads may submit meanwhile)."""
         _abi.check(_abi.lib().dop_server_run(self.handle), "dop_server_run")
+
+    def open(self):
+        """Keep run() waiting for submissions (until close())."""
+        _abi.check(_abi.lib().dop_server_set_open(self.handle, 1), "dop_server_set_open")
+
+    def close(self):
+        _abi.check(_abi.lib().dop_server_set_open(self.handle, 0), "dop_server_set_open")
+
+    def now(self):
+        """Seconds since the running run() started (-1 when not running)."""
+        return _abi.lib().dop_server_now(self.handle)
 
     def times(self):
         """(release, admitted, completed) seconds per request, relative to the run start."""

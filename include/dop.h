@@ -226,18 +226,32 @@ void dop_rk4_destroy(dop_rk4* rk);
  * A continuous-batching method-of-lines service.  Each request is a Q*u plan with its
  * scale, step dt, number of RK4 steps, an initial field and a result buffer in pinned host
  * memory, and a release time (seconds after dop_server_run starts).  dop_server_run
- * replays the releases in time, admits released requests in FIFO order while their state
- * (u and three RK4 stage buffers, carved from one device arena of `hbm_budget_bytes`) fits
- * and fewer than `max_active` are resident, advances every resident request by one RK4
- * step per iteration (each stage one grouped launch set over all of them), and copies each
- * finished state back asynchronously; requests that arrive while others run join at the
- * next iteration.  Storage f32 or f64 (one type per server). */
+ * releases requests at their times, admits released requests in FIFO order while their state
+ * (u and three RK4 stage buffers, carved from one device arena of `hbm_budget_bytes` rounded
+ * down to 256 bytes) fits and fewer than `max_active` are resident, and advances the resident
+ * requests by RK4 steps: they are split into size classes by grid points (up to 256^2, up to
+ * 1024^2, larger), each class on a stream of its own advancing all its requests one step per
+ * iteration (each stage one grouped launch set over the class), the classes side by side on
+ * the GPU, so a small request is not held to the pace of the largest resident grid.  A
+ * finished state is copied back asynchronously.  Storage f32 or f64 (one type per server).
+ * dop_server_submit may be called from any thread, also while dop_server_run runs: such a
+ * request joins when its release time (dop_server_now gives the run's current time) has
+ * passed.  dop_server_run returns once every submitted request has completed and the server
+ * is not open (dop_server_set_open(srv, 1) keeps it waiting for submissions until it is set
+ * to 0).  A request whose state alone exceeds the budget is refused at submission
+ * (DOP_EBUDGET); on an error the run stops and the resident requests' memory is returned.
+ * `stream` is unused (the classes run on the server's own streams, synchronized before
+ * dop_server_run returns); dop_server_times / dop_server_stats describe the last run and are
+ * read after it. */
 typedef struct dop_server dop_server;
 int dop_server_create(int64_t hbm_budget_bytes, int max_active, int chunk, void* stream,
                       dop_server** out);
 int dop_server_submit(dop_server* srv, const dop_plan* plan, double scale, double dt, int nsteps,
                       const void* u0_host, void* result_host, double release_s, int64_t* id);
-int dop_server_run(dop_server* srv);   /* returns when every submitted request completed */
+int dop_server_set_open(dop_server* srv, int open);
+int dop_server_run(dop_server* srv);
+/* seconds since the running dop_server_run started (-1 when it is not running) */
+double dop_server_now(const dop_server* srv);
 /* times (s, relative to the run start) of request `id`: release, admission, completion */
 int dop_server_times(const dop_server* srv, int64_t id, double* release, double* admitted,
                      double* completed);

@@ -25,7 +25,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libdop_oracle.so")
+_LIB_PATH = os.environ.get("DOP_ORACLE_LIBRARY") or os.path.join(_HERE, "_build", "libdop_oracle.so")
 
 COEFF_VECTOR, COEFF_NUM_POS, COEFF_NUM_NEG = 0, 1, 2
 

@@ -26,7 +26,8 @@ from diffeqoperators_amd.concretize import Solver, plan_csr
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HC_PATH = os.path.join(ROOT, "tests", "hostcheck", "_build", "libdop_hostcheck.so")
+HC_PATH = os.environ.get("DOP_HOSTCHECK_LIBRARY") or os.path.join(ROOT, "tests", "hostcheck", "_build",
+                                                                "libdop_hostcheck.so")
 
 
 def oop(L):

@@ -62,3 +62,76 @@ def test_nonuniform3d_512_autotuned_vs_oracle(dtype, tol):
     err = float(np.abs(got - ref).max() / np.abs(ref).max())
     print(f"  variant {variant}: normwise rel err {err:.3e}", flush=True)
     assert err <= tol
+
+
+def test_batched_1024_problem_mix_equals_single_applies():
+    # the 1024-problem mix of the batched workload (1-D / 2-D, N log-uniform up to 65536 / 2048,
+    # d in 1..4, a in {2,4,6,8}, Dirichlet / Neumann / Robin), FP32: the batched launch
+    # equals every problem's own apply bit for bit
+    from bench import batch_problem
+    probs = [batch_problem(i, torch.float32) for i in range(1024)]
+    print("case: 1024-problem batch vs single applies", flush=True)
+    dus = [torch.empty_like(p[1]) for p in probs]
+    b = D.Batch([p[0] for p in probs], [p[1] for p in probs], dus)
+    b.apply()
+    for i, p in enumerate(probs):
+        ref = torch.empty_like(p[1])
+        D.plans_for(p[0], p[1])[0].apply(ref, p[1])
+        assert torch.equal(dus[i], ref), i
+
+
+def test_batched_rk4_sample_vs_oracle():
+    # 16 problems of the mix advanced by the graph-captured RK4 (3 steps) against the FP64
+    # oracle's RK4 on the same FP32 initial values
+    from bench import batch_problem
+    idx = list(range(0, 1024, 64))
+    probs = [batch_problem(i, torch.float32) for i in idx]
+    u0 = [p[1].double().cpu().numpy() for p in probs]
+    rk = D.RK4([p[0] for p in probs], [p[1] for p in probs], [p[2] for p in probs],
+               [p[3] for p in probs])
+    rk.step(3)
+    torch.cuda.synchronize()
+    for k, (p, u) in enumerate(zip(probs, u0)):
+        A, sign, dt = p[0], p[2], p[3]
+        terms = A.ops if isinstance(A, D.DiffEqOperatorCombination) else [A]
+        oops, obcs = [], {}
+        for G in terms:
+            L = G.L
+            o = po.Op(int(L.wind), L.derivative_order, L.approximation_order, L.dx, L.len,
+                      axis=L.N - 1, offside=L.offside)
+            o.coeff = L.coefficients.copy()
+            o.coeff_kind = L.coeff_kind
+            oops.append(o)
+        nd = u.ndim
+        from diffeqoperators_amd.apply import _bcs_of
+        bcs, _ = _bcs_of(terms[0].Q, nd, tuple(reversed(u.shape)))
+        ob = [po.BC(bcs[a].a_l, bcs[a].b_l, bcs[a].a_r, bcs[a].b_r) for a in range(nd)]
+        f = lambda v: sign * po.ghost_apply(oops, ob, v)
+        for _ in range(3):
+            k1 = f(u)
+            k2 = f(u + 0.5 * dt * k1)
+            k3 = f(u + 0.5 * dt * k2)
+            k4 = f(u + dt * k3)
+            u = u + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        got = p[1].double().cpu().numpy()
+        err = np.abs(got - u).max() / np.abs(u).max()
+        assert err < 1e-5, (idx[k], err)
+
+
+def test_served_200_request_mix_sample_equals_alone():
+    # the served workload's 200 requests (Poisson arrivals at 2000/s, 64^2..2048^2, 100 RK4
+    # steps): a sample of 10 results equals the same request served alone, bit for bit
+    from bench import served_requests
+    reqs = served_requests(200, 2000.0)
+    plans = [D.plan_for_shape(A, (n, n), torch.float32) for A, n, _, _, _ in reqs]
+    srv = D.Server(8 << 30, max_active=4096, chunk=16)
+    for (A, n, u0, dt, rel), plan in zip(reqs, plans):
+        srv.submit(plan, u0, 1.0, dt, 100, rel)
+    print("case: 200-request served mix", flush=True)
+    srv.run()
+    for i in range(0, 200, 20):
+        A, n, u0, dt, _ = reqs[i]
+        alone = D.Server(1 << 30, max_active=4, chunk=16)
+        j = alone.submit(plans[i], u0, 1.0, dt, 100, 0.0)
+        alone.run()
+        assert torch.equal(srv.results[i], alone.results[j]), i

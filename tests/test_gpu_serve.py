@@ -92,3 +92,60 @@ def test_served_matches_oracle_rk4(requests):
     err = np.abs(got - u).max() / np.abs(u).max()
     print(f"served vs FP64 oracle RK4 (6 steps, FP32 storage): {err:.3e}")
     assert err < 1e-4
+
+
+def test_live_submission_while_running(requests):
+    # requests submitted from another thread while run() serves: each completes, after its
+    # release, with the result it gives alone
+    import threading
+    srv = D.Server(1 << 28, max_active=4, chunk=16)
+    srv.open()
+    th = threading.Thread(target=srv.run)
+    th.start()
+    ids, plans = [], []
+    import time
+    try:
+        for k, (A, _, u0, dt, _, n) in enumerate(requests[:6]):
+            plan = D.plan_for_shape(A, (n, n), torch.float32)
+            plans.append(plan)
+            ids.append(srv.submit(plan, u0, 1.0, dt, 8))   # released now
+            time.sleep(0.002)
+    finally:
+        srv.close()
+        th.join(timeout=60)
+    assert not th.is_alive()
+    t = srv.times()
+    assert np.all(t[:, 2] >= t[:, 0])
+    assert t[-1, 0] > 0.0     # the last ones were submitted after the run had started
+    for k, i in enumerate(ids):
+        r = requests[k]
+        _, alone = run_server([r[:4] + (0.0, r[5])])
+        assert torch.equal(srv.results[i], alone[0]), k
+
+
+def test_small_requests_keep_their_own_pace():
+    # two large grids and eight small ones released together: the small ones run in their own
+    # size class and finish before either large one
+    reqs = [advdiff(1100, 900), advdiff(1100, 901)] + [advdiff(64, 910 + k) for k in range(8)]
+    srv = D.Server(1 << 30, max_active=64, chunk=16)
+    ids = []
+    for (A, _, u0, dt), n in zip(reqs, [1100, 1100] + [64] * 8):
+        ids.append(srv.submit(D.plan_for_shape(A, (n, n), torch.float32), u0, 1.0, dt, 60, 0.0))
+    srv.run()
+    t = srv.times()
+    assert t[2:, 2].max() < t[:2, 2].min()
+
+
+def test_budget_is_checked_on_the_rounded_state():
+    A, _, u0, dt = advdiff(50, 77)
+    plan = D.plan_for_shape(A, (50, 50), torch.float32)
+    state = 4 * ((50 * 50 * 4 + 255) // 256 * 256)
+    small = D.Server(state - 1, max_active=4)       # rounds down below the state: refused
+    with pytest.raises(D._abi.DopError) as e:
+        small.submit(plan, u0, 1.0, dt, 2, 0.0)
+    assert e.value.code == D._abi.EBUDGET
+    ok = D.Server(state + 100, max_active=4)          # rounds down to exactly the state
+    ok.submit(plan, u0, 1.0, dt, 2, 0.0)
+    ok.submit(plan, u0, 1.0, dt, 2, 0.0)              # waits for the first one's memory
+    ok.run()
+    assert ok.stats()["peak_active"] == 1

@@ -17,7 +17,8 @@ from diffeqoperators_amd import _abi
 from diffeqoperators_amd.apply import _PeriodicPad, make_desc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HC_PATH = os.path.join(ROOT, "tests", "hostcheck", "_build", "libdop_hostcheck.so")
+HC_PATH = os.environ.get("DOP_HOSTCHECK_LIBRARY") or os.path.join(ROOT, "tests", "hostcheck", "_build",
+                                                                "libdop_hostcheck.so")
 
 
 def rel(a, b):

@@ -16,7 +16,8 @@ from diffeqoperators_amd import _abi
 from diffeqoperators_amd.apply import make_desc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HC_PATH = os.path.join(ROOT, "tests", "hostcheck", "_build", "libdop_hostcheck.so")
+HC_PATH = os.environ.get("DOP_HOSTCHECK_LIBRARY") or os.path.join(ROOT, "tests", "hostcheck", "_build",
+                                                                "libdop_hostcheck.so")
 
 
 def rel(a, b):
@@ -118,7 +119,7 @@ def test_tensor_api_has_no_cpu_path():
 def _declared_functions():
     text = open(os.path.join(ROOT, "include", "dop.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void|int64_t|const char\*)\s+\**(dop_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|int64_t|double|const char\*)\s+\**(dop_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_library_exports_every_declared_symbol():
