@@ -137,12 +137,12 @@ DOP_HD inline double ghost_left_f(const AxisDev& X, int64_t line, const F& ld)
 {
     if (X.periodic) return ld(X.periodic == 1 ? (int64_t)(X.n - 1) : 0);
     const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
-    if (e && e[0] < 0.0) return ld((int64_t)(X.n - 1));
-    const int nl = e ? (int)e[0] : X.nl;
+    if (e && tab(e, 0) < 0.0) return ld((int64_t)(X.n - 1));
+    const int nl = e ? (int)tab(e, 0) : X.nl;
     const double* al = e ? e + 2 : X.al;
     double g = 0.0;
-    for (int k = 0; k < nl; ++k) g = fma(al[k], ld(k), g);
-    return g + (e ? e[2 + kMaxBCTaps] : X.bl);
+    for (int k = 0; k < nl; ++k) g = fma(tab(al, k), ld(k), g);
+    return g + (e ? tab(e, 2 + kMaxBCTaps) : X.bl);
 }
 
 template <typename F>
@@ -150,13 +150,13 @@ DOP_HD inline double ghost_right_f(const AxisDev& X, int64_t line, const F& ld)
 {
     if (X.periodic) return ld(X.periodic == 2 ? (int64_t)(X.n - 1) : 0);
     const double* e = X.lbc ? X.lbc + line * kLineBC : nullptr;
-    if (e && e[1] < 0.0) return ld(0);
-    const int nr = e ? (int)e[1] : X.nr;
+    if (e && tab(e, 1) < 0.0) return ld(0);
+    const int nr = e ? (int)tab(e, 1) : X.nr;
     const double* ar = e ? e + 3 + kMaxBCTaps : X.ar;
     double g = 0.0;
     const int64_t o = (int64_t)X.n - nr;
-    for (int k = 0; k < nr; ++k) g = fma(ar[k], ld(o + k), g);
-    return g + (e ? e[3 + 2 * kMaxBCTaps] : X.br);
+    for (int k = 0; k < nr; ++k) g = fma(tab(ar, k), ld(o + k), g);
+    return g + (e ? tab(e, 3 + 2 * kMaxBCTaps) : X.br);
 }
 
 template <typename T>
@@ -189,8 +189,8 @@ DOP_HD inline double boundary_row_f(const PlanDev& P, const AxisDev& X, int r, i
                                     const F& ld)
 {
     const int b = r < X.rlo ? r : X.rlo + (r - X.rhi);
-    const int J0 = X.bJ0[b];
-    const int nt = X.bn[b];
+    const int J0 = tab(X.bJ0, b);
+    const int nt = tab(X.bn, b);
     const double* w = X.bw + (int64_t)b * X.KB;
     const int joff = P.ghost ? -1 : 0;
     const bool gl_used = P.ghost && J0 == 0;
@@ -211,7 +211,7 @@ DOP_HD inline double boundary_row_f(const PlanDev& P, const AxisDev& X, int r, i
         }
 #pragma unroll
         for (int k = 0; k < kEvalUnroll; ++k)
-            if (k < nt) s = fma(w[k], v[k], s);
+            if (k < nt) s = fma(tab(w, k), v[k], s);
         return s;
     }
     for (int k = 0; k < nt; ++k) {
@@ -220,7 +220,7 @@ DOP_HD inline double boundary_row_f(const PlanDev& P, const AxisDev& X, int r, i
         if (gl_used && J == 0) v = gl;
         else if (gr_used && J == X.n + 1) v = gr;
         else v = ld((int64_t)(J + joff));
-        s = fma(w[k], v, s);
+        s = fma(tab(w, k), v, s);
     }
     return s;
 }
@@ -252,7 +252,7 @@ DOP_HD inline void axis_term(const PlanDev& P, const AxisDev& X, const T* src, i
 #pragma unroll
             for (int k = 0; k < kEvalUnroll; ++k) {
                 if (k < X.K) {
-                    wv[k] = w[k * ws];
+                    wv[k] = tab(w, k * ws);
                     xv[k] = (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S);
                 }
             }
@@ -261,7 +261,7 @@ DOP_HD inline void axis_term(const PlanDev& P, const AxisDev& X, const T* src, i
                 if (k < X.K) acc.add(wv[k], xv[k]);
         } else {
             for (int k = 0; k < X.K; ++k)
-                acc.add(w[k * ws], (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
+                acc.add(tab(w, k * ws), (typename Acc<ACC>::W)Store<T>::ld(src, p0 + k * S));
         }
     } else {
         acc.add_sum(boundary_row<T>(P, X, src, r, base, S, line));

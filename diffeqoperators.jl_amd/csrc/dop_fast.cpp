@@ -42,6 +42,145 @@ int64_t shell_boxes(const PlanDev& P, const int32_t* lo, const int32_t* hi, Shel
     return total;
 }
 
+// Whether a shell strip that touches x boundary rows finds every x tap and ghost it reads in
+// the kStripStage values at its end of the line (left rows: [0, kStripStage), right rows:
+// [n - kStripStage, n)): strips are kStripPts aligned, so their points lie below
+// roundup(rlo) + kStripPts or at or above rhi - kStripPts.
+bool strips_stage_ok(const PlanDev& P, const AxisTables& T)
+{
+    const AxisDev& X = P.ax[0];
+    const int n = (int)P.n[0];
+    if (!P.ghost || !X.active || n < 2 * kStripStage || n % kStripPts != 0 || X.periodic) return false;
+    for (size_t l = 0; l < T.lbc.size(); l += kLineBC)
+        if (T.lbc[l] < 0.0) return false;   // a periodic line reads the far end
+    const int lend = std::min(n, (X.rlo + kStripPts - 1) / kStripPts * kStripPts + kStripPts);
+    const int rbeg = std::max(0, X.rhi - kStripPts);
+    auto within = [&](int r, int j) {
+        return r < lend ? (j >= 0 && j < kStripStage) : (j >= n - kStripStage && j < n);
+    };
+    for (int r = 0; r < n; ++r) {
+        if (r >= lend && r < rbeg) continue;
+        if (r >= X.rlo && r < X.rhi) {
+            for (int k = 0; k < X.K; ++k)
+                if (!within(r, r + X.dmin + k)) return false;
+            continue;
+        }
+        const int b = r < X.rlo ? r : X.rlo + (r - X.rhi);
+        for (int k = 0; k < T.bn[b]; ++k) {
+            const int J = T.bJ0[b] + k;
+            if (J == 0) {
+                for (int m = 0; m < X.nl; ++m)
+                    if (!within(r, m)) return false;
+            } else if (J == n + 1) {
+                for (int m = 0; m < X.nr; ++m)
+                    if (!within(r, n - X.nr + m)) return false;
+            } else if (!within(r, J - 1)) {
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+namespace {
+
+// The 3-D shell (the domain minus the box [lo, hi)) as work segments of the strip kernel; false
+// if a segment would not fit its 32-bit counts or the tables.
+bool build_strips(const PlanDev& P, const HostPlan& H, const int32_t* lo, const int32_t* hi,
+                  StripArgs* out)
+{
+    StripArgs& S = *out;
+    S = StripArgs{};
+    const int32_t n[3] = {(int32_t)P.n[0], (int32_t)P.n[1], (int32_t)P.n[2]};
+    const AxisDev& X0 = P.ax[0];
+    // every item starts a 16-byte group of u and du (inside the array) when n0, the row and
+    // plane strides, the transverse shifts and the x edges of the box are multiples of 4
+    bool al = n[0] % kStripPts == 0 && lo[0] % kStripPts == 0 && hi[0] % kStripPts == 0;
+    for (int a = 1; a < 3; ++a) al = al && P.ssrc[a] % kStripPts == 0 && P.sdu[a] % kStripPts == 0;
+    for (int a = 0; a < 3; ++a) al = al && P.tshift[a] % kStripPts == 0;
+    S.vec_ok = al ? 1 : 0;
+    S.stage_ok = strips_stage_ok(P, H.axis[0]) ? 1 : 0;
+    // x strips of four regular rows
+    const int32_t ax = (X0.rlo + kStripPts - 1) / kStripPts * kStripPts;
+    const int32_t bx = X0.rhi / kStripPts * kStripPts;
+    const bool faces = S.vec_ok && n[0] >= kStripStage && ax < bx;
+    const bool face[3] = {faces && S.stage_ok && !X0.lbc, faces && !P.ax[1].lbc, faces && !P.ax[2].lbc};
+
+    int nbox = 0;
+    auto add_seg = [&](int kind, int r0, int nr, int xend, int64_t items) {
+        if (nr <= 0 || items <= 0) return true;
+        if (S.nseg >= kMaxShellSegs || items >= ((int64_t)1 << 31)) return false;
+        ShellSeg& G = S.seg[S.nseg++];
+        G = ShellSeg{kind, r0, nr, xend, (int32_t)items, S.nblocks};
+        const int64_t nb = (int64_t)nr * ceil_div(items, kStripBlock);
+        if ((int64_t)S.nblocks + nb >= ((int64_t)1 << 31)) return false;
+        S.nblocks += (int32_t)nb;
+        return true;
+    };
+    auto add_box = [&](int32_t x0, int32_t x1, int32_t y0, int32_t y1, int32_t z0, int32_t z1) {
+        if (x1 <= x0 || y1 <= y0 || z1 <= z0) return true;
+        if (nbox >= kMaxStripBoxes) return false;
+        const int b = nbox++;
+        S.org[b][0] = x0; S.org[b][1] = y0; S.org[b][2] = z0;
+        S.ext[b][0] = x1 - x0; S.ext[b][1] = y1 - y0; S.ext[b][2] = z1 - z0;
+        S.nsx[b] = (int32_t)ceil_div(x1 - x0, kStripPts);
+        S.zfirst[b] = S.ext[b][2] <= S.ext[b][1] ? 1 : 0;
+        return add_seg(3, b, 1, 0, (int64_t)S.nsx[b] * S.ext[b][1] * S.ext[b][2]);
+    };
+    bool ok = true;
+    // x sides: y and z inside the box
+    S.flo[0][0] = lo[1]; S.fn[0][0] = hi[1] - lo[1];
+    S.flo[0][1] = lo[2]; S.fn[0][1] = hi[2] - lo[2];
+    // kind 4 (x faces in tiles of kXfTile^2 lines, one block each): the y and z windows must
+    // reach no further than the kXfHalo border lines
+    bool reach = true;
+    for (int a = 1; a < 3; ++a)
+        reach = reach && -P.ax[a].dmin <= kXfHalo && P.ax[a].dmin + P.ax[a].K - 1 <= kXfHalo;
+    const bool tiles = face[0] && reach && kXfTile * kXfTile == kStripBlock;
+    const int64_t xf_tiles = ceil_div(S.fn[0][0], kXfTile) * ceil_div(S.fn[0][1], kXfTile);
+    for (int side = 0; side < 2; ++side) {
+        const int32_t x0 = side ? hi[0] : 0, x1 = side ? n[0] : lo[0];
+        if (tiles) {
+            for (int32_t xs = x0; xs < x1; xs += kStripPts)
+                ok = ok && add_seg(4, xs, 1, x1, xf_tiles * kStripBlock);
+            S.nfirst = S.nblocks;
+        } else if (face[0]) {
+            ok = ok && add_seg(0, x0, (int32_t)ceil_div(x1 - x0, kStripPts), x1,
+                               (int64_t)S.fn[0][0] * S.fn[0][1]);
+        } else {
+            ok = ok && add_box(x0, x1, lo[1], hi[1], lo[2], hi[2]);
+        }
+    }
+    // z sides: all x and y
+    S.flo[2][0] = ax; S.fn[2][0] = (bx - ax) / kStripPts;
+    S.flo[2][1] = P.ax[1].rlo; S.fn[2][1] = P.ax[1].rhi - P.ax[1].rlo;
+    for (int side = 0; side < 2; ++side) {
+        const int32_t z0 = side ? hi[2] : 0, z1 = side ? n[2] : lo[2];
+        if (face[2] && S.fn[2][1] > 0) {
+            ok = ok && add_seg(2, z0, z1 - z0, 0, (int64_t)S.fn[2][0] * S.fn[2][1]);
+            ok = ok && add_box(0, ax, 0, n[1], z0, z1) && add_box(bx, n[0], 0, n[1], z0, z1);
+            ok = ok && add_box(ax, bx, 0, P.ax[1].rlo, z0, z1) && add_box(ax, bx, P.ax[1].rhi, n[1], z0, z1);
+        } else {
+            ok = ok && add_box(0, n[0], 0, n[1], z0, z1);
+        }
+    }
+    // y sides: all x, z inside the box
+    S.flo[1][0] = ax; S.fn[1][0] = (bx - ax) / kStripPts;
+    S.flo[1][1] = lo[2]; S.fn[1][1] = hi[2] - lo[2];
+    for (int side = 0; side < 2; ++side) {
+        const int32_t y0 = side ? hi[1] : 0, y1 = side ? n[1] : lo[1];
+        if (face[1]) {
+            ok = ok && add_seg(1, y0, y1 - y0, 0, (int64_t)S.fn[1][0] * S.fn[1][1]);
+            ok = ok && add_box(0, ax, y0, y1, lo[2], hi[2]) && add_box(bx, n[0], y0, y1, lo[2], hi[2]);
+        } else {
+            ok = ok && add_box(0, n[0], y0, y1, lo[2], hi[2]);
+        }
+    }
+    return ok;
+}
+
 }  // namespace
 
 void build_fast(const HostPlan& H, FastPlan* fp)
@@ -114,24 +253,7 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     }
     // shells: the domain minus the box, slabs from the slowest axis down
     fp->shell_total = shell_boxes(P, A.lo, A.hi, &fp->shell);
-    {
-        StripArgs& S = fp->strips;
-        S = StripArgs{};
-        S.nbox = fp->shell.nbox;
-        S.vec_ok = (P.n[0] % kStripPts == 0) ? 1 : 0;
-        int64_t tot = 0;
-        for (int b = 0; b < S.nbox; ++b) {
-            for (int k = 0; k < 3; ++k) {
-                S.org[b][k] = fp->shell.org[b][k];
-                S.ext[b][k] = fp->shell.ext[b][k];
-            }
-            S.nsx[b] = ceil_div(S.ext[b][0], kStripPts);
-            S.xcol[b] = S.ext[b][0] < P.n[0] ? 1 : 0;
-            S.start[b] = tot;
-            tot += S.nsx[b] * S.ext[b][1] * S.ext[b][2];
-        }
-        S.start[S.nbox] = tot;
-    }
+    if (P.ndim == 3 && !build_strips(P, H, A.lo, A.hi, &fp->strips)) return;
     (void)shell_boxes(P, fp->nat_lo, fp->nat_hi, &fp->nat_shell);
     fp->kind = P.ndim;
 }

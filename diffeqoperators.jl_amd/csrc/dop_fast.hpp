@@ -20,18 +20,50 @@ struct ShellArgs {
     int64_t start[kMaxBoxes + 1];   // prefix counts; start[nbox] = total
 };
 
-// The shell as strips of up to kStripPts consecutive x points of one row (the 3-D shell
-// kernel's work items): per box, its origin, extent, strips along x and prefix counts.
+// The 3-D shell as work items of up to kStripPts consecutive x points of one row, in
+// segments of whole blocks of the shell kernel:
+//  - face segments (kind a = 0, 1, 2): the shell rows of axis a on one side of the box, over
+//    the part of the face whose other two axes are regular rows (kind 1, 2: x strips of four
+//    regular rows; kind 0: the strips at one x end of the lines).  Within a block every item
+//    has the same row along a, so the row's program is uniform; the rows of a segment take
+//    turns block by block (items sharing their taps run together);
+//  - general segments (kind 3): one box of the rest (edges, ends of lines, per-line BCs, ...).
 constexpr int kStripPts = 4;
-struct StripArgs {
-    int32_t nbox;
-    int32_t vec_ok;                 // n0 % 4 == 0: strips starting at x % 4 == 0 are aligned
-    int64_t org[kMaxBoxes][3];
-    int64_t ext[kMaxBoxes][3];
-    int64_t nsx[kMaxBoxes];         // strips along x
-    int64_t start[kMaxBoxes + 1];   // prefix counts of strips; start[nbox] = total
-    int32_t xcol[kMaxBoxes];        // 1: the box holds x boundary columns (stage the line end)
+constexpr int kStripBlock = 256;    // items per block of the shell kernel
+constexpr int kMaxStripBoxes = 16;
+constexpr int kMaxShellSegs = 32;
+struct ShellSeg {
+    int32_t kind;    // 0, 1, 2: face rows of that axis; 3: general strips of box r0; 4: the x
+                     // face strip starting at x0 = r0 in tiles of kXfTile^2 lines (one block each)
+    int32_t r0;      // face: first row (kind 0: first strip start x0); general: box
+    int32_t nr;      // face: rows r0 .. r0 + nr - 1 (kind 0: strip starts 4 apart); general: 1
+    int32_t xend;    // kind 0: end of the x range (its last strip may be narrower)
+    int32_t items;   // work items per row (general: strips of the box)
+    int32_t first;   // first block (in the order the XCDs take them)
 };
+struct StripArgs {
+    int32_t nseg;
+    int32_t nblocks;                 // blocks of the launch
+    int32_t nfirst;                  // blocks of the leading kind-4 segments (taken in hardware
+                                     // order; the rest XCD-contiguous)
+    int32_t vec_ok;                  // every item starts an aligned group of 4 values of u and du
+    int32_t stage_ok;                // every x tap and ghost of a strip touching x boundary rows
+                                     // lies in the kStripStage values at its end of the line
+    ShellSeg seg[kMaxShellSegs];
+    // general boxes: origin, extent, strips along x, order (zfirst: x, z, y -- a box thin
+    // along z -- instead of x, y, z)
+    int32_t org[kMaxStripBoxes][3];
+    int32_t ext[kMaxStripBoxes][3];
+    int32_t nsx[kMaxStripBoxes];
+    int32_t zfirst[kMaxStripBoxes];
+    // faces of kind a: the item's coordinates (u fastest, v): kind 2 (x strip, y), kind 1
+    // (x strip, z), kind 0 (y, z); flo = first x (strips) or row, fn = count
+    int32_t flo[3][2];
+    int32_t fn[3][2];
+};
+constexpr int kXfTile = 16;     // kind 4: lines per tile side (y and z); kXfTile^2 = block size
+constexpr int kXfHalo = 4;      // kind 4: border lines staged on each side (>= the y and z reach)
+constexpr int kStripStage = 16;
 
 // The interior box and the per-axis weights of the tiled kernels.  Every active axis uses
 // one symmetric odd window of K taps (narrower operator windows are padded with zero

@@ -21,6 +21,19 @@
 
 namespace dop {
 
+// Element i of a plan table.  Every plan (PlanDev, its AxisDev and the tables they point to)
+// lives in device global memory, so device code reads it with global rather than flat loads
+// (flat loads also count against the LDS wait counter).
+template <typename V>
+DOP_HD inline V tab(const V* p, int64_t i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(1))) V*)p)[i];
+#else
+    return p[i];
+#endif
+}
+
 constexpr int kMaxBCTaps = 8;
 constexpr int kMaxDim = 3;
 

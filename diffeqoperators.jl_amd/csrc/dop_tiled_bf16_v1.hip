@@ -5,9 +5,9 @@
 namespace dop {
 
 hipError_t launch_tiled_bf16_v1(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
-                             int accumulate, hipStream_t s)
+                             int accumulate, hipStream_t s, hipStream_t s2)
 {
-    return tiled::launch_interior_k<bf16, 1>(p, A, du, u, accumulate, s);
+    return tiled::launch_interior_k<bf16, 1>(p, A, du, u, accumulate, s, s2);
 }
 
 }  // namespace dop

@@ -945,8 +945,8 @@ __global__ __launch_bounds__(64 * W) void k_tiled3e(const FastArgs A, T* __restr
 
 // ---------------------------------------------------------------------------------------
 template <typename T, int VX, int K>
-hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
-                           int accumulate, hipStream_t s)
+hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
+                                 int accumulate, hipStream_t s)
 {
     const FastPlan& fp = p.fast;
     const bool uni = A.uniform[0] && A.uniform[1] && (fp.kind == 2 || A.uniform[2]);
@@ -956,15 +956,6 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
     const int64_t per = (int64_t)256 * kShellPtsPerThread;
     // 2-D: the (thin) shell rides in the tile launch; 3-D: a launch of its own first
     const int nshell = fp.kind == 2 ? (int)((fp.shell_total + per - 1) / per) : 0;
-    if (fp.kind == 3 && fp.shell_total > 0) {
-        const int64_t items = fp.strips.start[fp.strips.nbox];
-        hipLaunchKernelGGL((shell::k_shell_strips<T, acc_of<T>()>),
-                           dim3((unsigned)((items + shell::kShellBlock - 1) / shell::kShellBlock)),
-                           dim3(shell::kShellBlock), 0, s, p.dev, fp.strips, (T*)du, (const T*)u,
-                           accumulate);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
     if (fp.kind == 3 && A.gen == 2) {
         if constexpr (K <= 9 && VX <= 16 / (int)sizeof(T) && VX <= 4) {
 #define DOP_L3E(UNI, W, VY)                                                                  \
@@ -1033,17 +1024,51 @@ hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, con
     return hipGetLastError();
 }
 
+// The 3-D shell (its own launch, on s2; may be s)
+template <typename T, int K>
+hipError_t launch_shell3d(const DevicePlan& p, void* du, const void* u, int accumulate, hipStream_t s2)
+{
+    const FastPlan& fp = p.fast;
+    if constexpr (K <= 9) {
+        if (fp.kind == 3 && fp.strips.nblocks > 0) {
+            const dim3 sg((unsigned)fp.strips.nblocks);
+            const StripArgs& SA = fp.strips;
+            if (SA.vec_ok)
+                hipLaunchKernelGGL((shell::k_shell_strips<T, acc_of<T>(), K, true>), sg,
+                                   dim3(shell::kShellBlock), 0, s2, p.dev, SA, (T*)du, (const T*)u,
+                                   accumulate);
+            else
+                hipLaunchKernelGGL((shell::k_shell_strips<T, acc_of<T>(), K, false>), sg,
+                                   dim3(shell::kShellBlock), 0, s2, p.dev, SA, (T*)du, (const T*)u,
+                                   accumulate);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// The tiles on s, then (3-D) the shell on s2: the long bandwidth-bound launch is dispatched first
+template <typename T, int VX, int K>
+hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
+                           int accumulate, hipStream_t s, hipStream_t s2)
+{
+    const hipError_t e = launch_interior_tiles<T, VX, K>(p, A, du, u, accumulate, s);
+    if (e != hipSuccess || p.fast.kind != 3) return e;
+    return launch_shell3d<T, K>(p, du, u, accumulate, s2);
+}
+
 template <typename T, int VX>
 hipError_t launch_interior_k(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
-                             int accumulate, hipStream_t s)
+                             int accumulate, hipStream_t s, hipStream_t s2)
 {
     switch (A.K) {
-    case 3: return launch_interior<T, VX, 3>(p, A, du, u, accumulate, s);
-    case 5: return launch_interior<T, VX, 5>(p, A, du, u, accumulate, s);
-    case 7: return launch_interior<T, VX, 7>(p, A, du, u, accumulate, s);
-    case 9: return launch_interior<T, VX, 9>(p, A, du, u, accumulate, s);
-    case 11: return launch_interior<T, VX, 11>(p, A, du, u, accumulate, s);
-    case 13: return launch_interior<T, VX, 13>(p, A, du, u, accumulate, s);
+    case 3: return launch_interior<T, VX, 3>(p, A, du, u, accumulate, s, s2);
+    case 5: return launch_interior<T, VX, 5>(p, A, du, u, accumulate, s, s2);
+    case 7: return launch_interior<T, VX, 7>(p, A, du, u, accumulate, s, s2);
+    case 9: return launch_interior<T, VX, 9>(p, A, du, u, accumulate, s, s2);
+    case 11: return launch_interior<T, VX, 11>(p, A, du, u, accumulate, s, s2);
+    case 13: return launch_interior<T, VX, 13>(p, A, du, u, accumulate, s, s2);
     }
     return hipErrorInvalidValue;
 }

@@ -8,6 +8,8 @@
 #include "../../diffeqoperators.jl_amd/csrc/dop_build.hpp"
 #include "../../diffeqoperators.jl_amd/csrc/dop_concrete.hpp"
 #include "../../diffeqoperators.jl_amd/csrc/dop_eval.hpp"
+#include "../../diffeqoperators.jl_amd/csrc/dop_fast.hpp"
+#include "../../diffeqoperators.jl_amd/csrc/dop_strip.hpp"
 
 namespace {
 thread_local std::string g_err;
@@ -90,7 +92,96 @@ int hc_apply(const dop_desc* d, void* du, const void* u, int accumulate)
 }
 }
 
+namespace {
+template <typename T, int ACC, int KM>
+void run_shell_k(const dop::PlanDev& P, const dop::StripArgs& S, T* du, const T* src, int accumulate)
+{
+    T stage[dop::strip::kStage + 1];
+    dop::strip::Slot<ACC> slots[dop::strip::kSW];
+    for (int s = 0; s < S.nseg; ++s) {
+        const dop::ShellSeg& G = S.seg[s];
+        if (G.kind == 4) {
+            // the device marches these x-face points along z; the same points as face items
+            for (int l = 0; l < S.fn[0][0] * S.fn[0][1]; ++l)
+                dop::strip::eval_face<T, ACC, KM, 0, dop::strip::ScalarLoad<T>>(P, S, G.r0, G.xend, l, stage, slots,
+                                                                               du, src, accumulate);
+            continue;
+        }
+        for (int j = 0; j < G.nr; ++j) {
+            for (int l = 0; l < G.items; ++l) {
+                if (S.vec_ok)
+                    dop::strip::eval_item<T, ACC, KM, true, dop::strip::ScalarLoad<T>>(P, S, s, j, l, stage, slots,
+                                                                                     du, src, accumulate);
+                else
+                    dop::strip::eval_item<T, ACC, KM, false, dop::strip::ScalarLoad<T>>(P, S, s, j, l, stage,
+                                                                                      slots, du, src, accumulate);
+            }
+        }
+    }
+}
+
+// the tiles' window K (3, 5, 7 or 9) as the strips' compile-time width, as the launch does
+template <typename T, int ACC>
+void run_shell(const dop::PlanDev& P, const dop::StripArgs& S, int K, T* du, const T* src, int accumulate)
+{
+    switch (K) {
+    case 3: run_shell_k<T, ACC, 3>(P, S, du, src, accumulate); break;
+    case 5: run_shell_k<T, ACC, 5>(P, S, du, src, accumulate); break;
+    case 7: run_shell_k<T, ACC, 7>(P, S, du, src, accumulate); break;
+    default: run_shell_k<T, ACC, 9>(P, S, du, src, accumulate); break;
+    }
+}
+}  // namespace
+
 extern "C" {
+// The 3-D shell of the tiled path (every point outside the tiles' box) evaluated strip by strip
+// with the device kernel's per-strip code; other points of du are left as they are.  Returns the
+// number of shell points, or -1 when the plan has no 3-D tiled path.
+int64_t hc_shell(const dop_desc* d, void* du, const void* u, int accumulate)
+{
+    dop::HostPlan H;
+    if (dop::build_host_plan(*d, &H, &g_err) != DOP_OK) return -1;
+    dop::FastPlan F;
+    dop::build_fast(H, &F);
+    if (F.kind != 3) return -1;
+    const dop::PlanDev P = bind(H);
+    using namespace dop;
+    switch (P.storage) {
+    case kF64: run_shell<double, kAccF64>(P, F.strips, F.args.K, (double*)du, (const double*)u, accumulate); break;
+    case kF32: run_shell<float, kAccF32>(P, F.strips, F.args.K, (float*)du, (const float*)u, accumulate); break;
+    case kBF16: run_shell<bf16, kAccF32>(P, F.strips, F.args.K, (bf16*)du, (const bf16*)u, accumulate); break;
+    }
+    return F.shell_total;
+}
+
+// Shell points handled per segment kind of the 3-D tiled plan (kinds 0, 1, 2: faces of that
+// axis; 3: general boxes) into pts[4]; -1 when the plan has no 3-D tiled path.
+int hc_shell_kinds(const dop_desc* d, int64_t* pts)
+{
+    dop::HostPlan H;
+    if (dop::build_host_plan(*d, &H, &g_err) != DOP_OK) return -1;
+    dop::FastPlan F;
+    dop::build_fast(H, &F);
+    if (F.kind != 3) return -1;
+    const dop::StripArgs& S = F.strips;
+    for (int k = 0; k < 4; ++k) pts[k] = 0;
+    for (int s = 0; s < S.nseg; ++s) {
+        const dop::ShellSeg& G = S.seg[s];
+        if (G.kind == 3) {
+            const int b = G.r0;
+            pts[3] += (int64_t)S.ext[b][0] * S.ext[b][1] * S.ext[b][2];
+        } else if (G.kind == 0) {
+            pts[0] += (int64_t)(G.xend - G.r0) * G.items;
+        } else if (G.kind == 4) {
+            const int w = G.xend - G.r0 < dop::kStripPts ? G.xend - G.r0 : dop::kStripPts;
+            pts[0] += (int64_t)w * S.fn[0][0] * S.fn[0][1];
+        } else {
+            pts[G.kind] += (int64_t)G.nr * G.items * dop::kStripPts;
+        }
+    }
+    return S.nseg;
+}
+
 // sizes of the ABI structs as the C++ compiler lays them out (checked against ctypes)
 int64_t hc_sizeof(int which)
 {
