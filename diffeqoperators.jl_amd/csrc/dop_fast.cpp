@@ -284,13 +284,13 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
     A.gen = 1;
     if (fp.kind == 3 && v >= gen1_count(fp)) {
         // edge-loaded segments: chunk depth {64,32} x block shape (waves, rows per wave)
-        // {(8,1),(8,2),(16,1),(16,2)} x XCD remap x points per lane {widest, half}
+        // {(8,1),(8,2),(16,1),(8,3)} x XCD remap x points per lane {widest, half}
         const int w = v - gen1_count(fp);
         A.gen = 2;
         A.xcd_remap = (w & 1) ? 0 : 1;
         static const int chunks[2] = {64, 32};
         A.chunk = chunks[(w >> 1) & 1];
-        static const int shapes[4][2] = {{8, 1}, {8, 2}, {16, 1}, {16, 2}};
+        static const int shapes[4][2] = {{8, 1}, {8, 2}, {16, 1}, {8, 3}};
         A.W = shapes[(w >> 2) & 3][0];
         A.VY = shapes[(w >> 2) & 3][1];
         A.VX = (w >> 4) ? fp.VX / 2 : fp.VX;

@@ -833,29 +833,35 @@ __global__ __launch_bounds__(64 * W) void k_tiled3e(const FastArgs A, T* __restr
     const int zlast = min(z1 + H - 1, n2 - 1);
     auto zc = [&](int zz) { return (int64_t)min(max(zz, 0), zlast) * s2; };
 
+    // halo row and edge group: a ring of D planes (D divides R, so a slot is a compile-time
+    // function of the step), loaded D - 1 planes ahead
+    constexpr int D = (R % 4 == 0) ? 4 : 2;
     C q[VY][R][VX];
-    C hq[2][VX];   // halo row, planes z and z + 1
-    C eq[2][VX];   // edge group, planes z and z + 1
+    C hq[D][VX];   // halo row, planes z .. z + D - 1
+    C eq[D][VX];   // edge group, planes z .. z + D - 1
 #pragma unroll
     for (int s = 0; s < R - 1; ++s) {
 #pragma unroll
         for (int i = 0; i < VY; ++i) VecIO<T, VX>::ld(urow[i] + zc(z0 - H + s), q[i][s]);
     }
-    VecIO<T, VX>::ld(uhalo + zc(z0), hq[0]);
-    VecIO<T, VX>::ld(uedge + zc(z0), eq[0]);
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s) {
+        VecIO<T, VX>::ld(uhalo + zc(z0 + s), hq[s]);
+        VecIO<T, VX>::ld(uedge + zc(z0 + s), eq[s]);
+    }
 
     int buf = 0;
     for (int zb = z0; zb < z1; zb += R) {
         auto step = [&](auto jc) -> bool {
             constexpr int j = decltype(jc)::value;
-            constexpr int cur = j & 1, nxt = (j + 1) & 1;
+            constexpr int cur = j % D, nxt = (j + D - 1) % D;
             const int z = zb + j;
             if (z >= z1) return false;   // block-uniform
 #pragma unroll
             for (int i = 0; i < VY; ++i)
                 VecIO<T, VX>::ld(urow[i] + zc(z + H + PF), q[i][(j + K - 1 + PF) % R]);
-            VecIO<T, VX>::ld(uhalo + zc(z + 1), hq[nxt]);
-            VecIO<T, VX>::ld(uedge + zc(z + 1), eq[nxt]);
+            VecIO<T, VX>::ld(uhalo + zc(z + D - 1), hq[nxt]);
+            VecIO<T, VX>::ld(uedge + zc(z + D - 1), eq[nxt]);
             // stage plane z: computed rows, the halo row, the edge groups
 #pragma unroll
             for (int i = 0; i < VY; ++i) {
@@ -969,7 +975,7 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
             if (shape == 8 * 8 + 1) DOP_L3EU(8, 1);
             else if (shape == 8 * 8 + 2) DOP_L3EU(8, 2);
             else if (shape == 16 * 8 + 1) DOP_L3EU(16, 1);
-            else if (shape == 16 * 8 + 2) DOP_L3EU(16, 2);
+            else if (shape == 8 * 8 + 3) DOP_L3EU(8, 3);
             else return hipErrorInvalidValue;
 #undef DOP_L3EU
 #undef DOP_L3E
