@@ -412,7 +412,8 @@ def batch_problem(i, dtype):
 
 
 def bench_batched(steps, nprob=1024, plain_sample=48, plain_steps=4):
-    """1024 independent problems, graph-captured RK4, one persistent launch per stage."""
+    """1024 independent problems, graph-captured RK4: each stage is a few grouped launches (one
+    per kind of problem), spread over four streams inside the graph."""
     import torch
     import diffeqoperators_amd as D
     probs = [batch_problem(i, torch.float32) for i in range(nprob)]

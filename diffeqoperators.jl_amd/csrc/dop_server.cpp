@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -254,9 +256,11 @@ int class_step(dop_server* srv, Class& C, size_t sb, std::vector<Request*>* drai
         C.dirty = false;
     }
     TableSlot& S = C.slots[C.cur_slot];
+    // one block per work item (no persistent blocks): blocks retire all the time, so the
+    // classes' streams share the CUs instead of the largest class holding them for a whole step
     for (int stage = 0; stage < 4; ++stage)
         if (dop::launch_batch_items(srv->storage, S.d_probs, S.d_items, S.groups, stage, 0,
-                                    srv->grid, C.st) != hipSuccess)
+                                    INT32_MAX, C.st) != hipSuccess)
             return dop::set_error(DOP_EHIP, "server: stage launch failed");
     (void)hipEventRecord(S.used, C.st);
     S.armed = true;
