@@ -192,8 +192,9 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     const bool f32 = (P.storage == kF32 || P.storage == kBF16) &&
                      (P.accum == kAccF32 || P.accum == kAccF32Kahan || P.accum == kAccF32Wide);
     if (!f64 && !f32) return;
-    // compensated modes: batched tiles only (single plans take the generic kernel)
-    fp->kahan = (P.accum == kAccF32Kahan || P.accum == kAccF32Wide) ? 1 : 0;
+    // the Dot2 compensated mode: batched tiles only (single plans take the generic kernel);
+    // f32 values summed in f64 (DOP_ACC_F32_WIDE) run the single-plan tiles too
+    fp->kahan = (P.accum == kAccF32Kahan) ? 1 : 0;
     // one symmetric window for all axes
     int half = 0;
     for (int a = 0; a < P.ndim; ++a) {

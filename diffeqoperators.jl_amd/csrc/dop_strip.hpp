@@ -55,15 +55,15 @@ struct ScalarLoad {
 
 // An accumulator parked in memory (LDS on the device): the loops that take the points of a
 // strip one at a time (x boundary rows, per-line BCs) index points at run time, which the
-// registers holding acc[] cannot be.  The shell runs the plain f64 / f32 sums, whose state is
-// one value.
+// registers holding acc[] cannot be.  The shell runs the f64, f32 and f32-values-in-f64 sums,
+// whose state is one value.
 template <int ACC>
-using Slot = typename Acc<ACC>::W;
+using Slot = decltype(Acc<ACC>::s);
 
 template <int ACC>
 DOP_HD inline void slot_put(Slot<ACC>* sl, int i, const Acc<ACC>& a)
 {
-    static_assert(ACC == kAccF64 || ACC == kAccF32, "one-value accumulators only");
+    static_assert(ACC == kAccF64 || ACC == kAccF32 || ACC == kAccF32Wide, "one-value accumulators only");
     sl[i] = a.s;
 }
 

@@ -186,6 +186,36 @@ __device__ inline double fmac<double>(double a, double b, double c) { return fma
 template <>
 __device__ inline float fmac<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
+// The running sum of a point: the storage's compute type C, or (ACC = kAccF32Wide) f32 weights
+// and values with products formed and summed in f64 -- the generic evaluator's Acc<ACC>
+// arithmetic, so tiled and generic results stay bitwise equal.
+template <typename C, int ACC>
+using Sum = typename std::conditional<ACC == kAccF32Wide, double, C>::type;
+
+template <typename C, int ACC>
+__device__ inline Sum<C, ACC> fsum(C w, C x, Sum<C, ACC> s)
+{
+    return fmac<Sum<C, ACC>>((Sum<C, ACC>)w, (Sum<C, ACC>)x, s);
+}
+
+// VX sums stored as T (plus the old du when accumulating, added in f64 like the generic store)
+template <typename T, int VX, int ACC>
+__device__ inline void store_sums(T* po, const Sum<typename Store<T>::C, ACC>* sums, int accumulate)
+{
+    using C = typename Store<T>::C;
+    C out[VX];
+    if (accumulate) {
+        C old[VX];
+        VecIO<T, VX>::ld(po, old);
+#pragma unroll
+        for (int v = 0; v < VX; ++v) out[v] = (C)((double)sums[v] + (double)old[v]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) out[v] = (C)sums[v];
+    }
+    VecIO<T, VX>::st(po, out);
+}
+
 template <typename C>
 __device__ inline const C* wtab(const FastArgs& A, int a);
 template <>
@@ -328,7 +358,7 @@ __device__ inline void x_window_dpp(const typename Store<T>::C* own, typename St
 // from neighbouring lanes through DPP, and only the middle lanes store.  Loads are clamped
 // into the array and issued on every path, stores are one predicated vector store, so the
 // in-order load counter waits stay precise and the ring stays in flight.
-template <typename T, int VX, int K, bool UNI, int PF>
+template <typename T, int VX, int K, bool UNI, int PF, int ACC>
 __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev* __restrict__ Pp,
                                                 const ShellArgs SH, int nshell,
                                                 T* __restrict__ du, const T* __restrict__ u,
@@ -343,7 +373,7 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     int bi;
     if (!block_role((int)blockIdx.x, A.nblocks, nshell, &bi)) {
         const int64_t per = 256 * kShellPtsPerThread;
-        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)bi * per, per);
+        shell_points<T, ACC>(*Pp, SH, du, u, accumulate, (int64_t)bi * per, per);
         return;
     }
     const int lane = threadIdx.x & 63;
@@ -403,26 +433,17 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
             C wy[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) wy[k] = UNI ? wyu[k] : wyt[(int64_t)y * K + k];
-            C out[VX];
+            Sum<C, ACC> out[VX];
 #pragma unroll
             for (int v = 0; v < VX; ++v) {
-                C s = C(0);
+                Sum<C, ACC> s = 0;
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fmac<C>(wx[v][k], xw[v + k], s);
+                for (int k = 0; k < K; ++k) s = fsum<C, ACC>(wx[v][k], xw[v + k], s);
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fmac<C>(wy[k], q[(j + k) % R][v], s);
+                for (int k = 0; k < K; ++k) s = fsum<C, ACC>(wy[k], q[(j + k) % R][v], s);
                 out[v] = s;
             }
-            if (st_ok) {
-                T* po = du + (int64_t)y * s1 + xs;
-                if (accumulate) {
-                    C old[VX];
-                    VecIO<T, VX>::ld(po, old);
-#pragma unroll
-                    for (int v = 0; v < VX; ++v) out[v] = (C)((double)out[v] + (double)old[v]);
-                }
-                VecIO<T, VX>::st(po, out);
-            }
+            if (st_ok) store_sums<T, VX, ACC>(du + (int64_t)y * s1 + xs, out, accumulate);
             return true;
         };
         if (!ring_steps<0, R>(step)) break;
@@ -448,7 +469,7 @@ constexpr int kMaxChunk3D = 64;
 template <typename T, int K>
 constexpr int kPF3D = sizeof(T) == 8 ? 1 : K <= 3 ? 4 : K <= 7 ? 2 : 1;
 
-template <typename T, int VX, int K, bool UNI, int PF, int W, int VY>
+template <typename T, int VX, int K, bool UNI, int PF, int W, int VY, int ACC>
 __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const PlanDev* __restrict__ Pp,
                                                      const ShellArgs SH, int nshell,
                                                      T* __restrict__ du, const T* __restrict__ u,
@@ -470,7 +491,7 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
     int bi;
     if (!block_role((int)blockIdx.x, A.nblocks, nshell, &bi)) {   // whole block: no barrier
         const int64_t per = 64 * W * kShellPtsPerThread;
-        shell_points<T, acc_of<T>()>(*Pp, SH, du, u, accumulate, (int64_t)bi * per, per);
+        shell_points<T, ACC>(*Pp, SH, du, u, accumulate, (int64_t)bi * per, per);
         return;
     }
 
@@ -588,34 +609,26 @@ __global__ __launch_bounds__(64 * W) void k_tiled3d(const FastArgs A, const Plan
 #pragma unroll
                     for (int v = 0; v < VX; ++v) xw[r * VX + v] = lds[buf][row[i]][wcol + r * VX + v];
                 }
-                C out[VX];
+                Sum<C, ACC> out[VX];
 #pragma unroll
                 for (int v = 0; v < VX; ++v) {
-                    C s = C(0);
+                    Sum<C, ACC> s = 0;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         C w;
                         if constexpr (UNI) w = wxu[k];
                         else w = wxs[k][lane * VX + v];
-                        s = fmac<C>(w, xw[PADL - H + v + k], s);
+                        s = fsum<C, ACC>(w, xw[PADL - H + v + k], s);
                     }
 #pragma unroll
                     for (int k = 0; k < K; ++k)
-                        s = fmac<C>(wy[i][k], lds[buf][row[i] - H + k][lane * VX + v], s);
+                        s = fsum<C, ACC>(wy[i][k], lds[buf][row[i] - H + k][lane * VX + v], s);
 #pragma unroll
-                    for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[i][(j + k) % R][v], s);
+                    for (int k = 0; k < K; ++k) s = fsum<C, ACC>(wz[k], q[i][(j + k) % R][v], s);
                     out[v] = s;
                 }
-                if (st_ok[i]) {
-                    T* po = du + (int64_t)z * s2 + (int64_t)y[i] * s1 + xs;
-                    if (accumulate) {
-                        C old[VX];
-                        VecIO<T, VX>::ld(po, old);
-#pragma unroll
-                        for (int v = 0; v < VX; ++v) out[v] = (C)((double)out[v] + (double)old[v]);
-                    }
-                    VecIO<T, VX>::st(po, out);
-                }
+                if (st_ok[i])
+                    store_sums<T, VX, ACC>(du + (int64_t)z * s2 + (int64_t)y[i] * s1 + xs, out, accumulate);
             }
             buf ^= 1;
             return true;
@@ -719,7 +732,7 @@ __device__ inline void x_window_edges(const C* own, const C (*eg)[VX], C* xw)
     }
 }
 
-template <typename T, int VX, int K, bool UNI, int PF, int W, int VY>
+template <typename T, int VX, int K, bool UNI, int PF, int W, int VY, int ACC>
 __global__ __launch_bounds__(64 * W) void k_tiled3e(const FastArgs A, T* __restrict__ du,
                                                      const T* __restrict__ u, int accumulate)
 {
@@ -910,37 +923,29 @@ __global__ __launch_bounds__(64 * W) void k_tiled3e(const FastArgs A, T* __restr
                 }
                 C xw[VX + 2 * H];
                 x_window_edges<C, VX, H>(c, eg, xw);
-                C out[VX];
+                Sum<C, ACC> out[VX];
 #pragma unroll
                 for (int v = 0; v < VX; ++v) {
-                    C s = C(0);
+                    Sum<C, ACC> s = 0;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         C w;
                         if constexpr (UNI) w = wxu[k];
                         else if constexpr (XWL) w = wxl[k][v];
                         else w = wx[v][k];
-                        s = fmac<C>(w, xw[v + k], s);
+                        s = fsum<C, ACC>(w, xw[v + k], s);
                     }
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         const C val = k == H ? c[v] : lds[buf][rr[i] - H + k][C0 + lane * VX + v];
-                        s = fmac<C>(wy[i][k], val, s);
+                        s = fsum<C, ACC>(wy[i][k], val, s);
                     }
 #pragma unroll
-                    for (int k = 0; k < K; ++k) s = fmac<C>(wz[k], q[i][(j + k) % R][v], s);
+                    for (int k = 0; k < K; ++k) s = fsum<C, ACC>(wz[k], q[i][(j + k) % R][v], s);
                     out[v] = s;
                 }
-                if (st_ok[i]) {
-                    T* po = du + (int64_t)z * s2 + (int64_t)y[i] * s1 + xs;
-                    if (accumulate) {
-                        C old[VX];
-                        VecIO<T, VX>::ld(po, old);
-#pragma unroll
-                        for (int v = 0; v < VX; ++v) out[v] = (C)((double)out[v] + (double)old[v]);
-                    }
-                    VecIO<T, VX>::st(po, out);
-                }
+                if (st_ok[i])
+                    store_sums<T, VX, ACC>(du + (int64_t)z * s2 + (int64_t)y[i] * s1 + xs, out, accumulate);
             }
             buf ^= 1;
             return true;
@@ -950,7 +955,7 @@ __global__ __launch_bounds__(64 * W) void k_tiled3e(const FastArgs A, T* __restr
 }
 
 // ---------------------------------------------------------------------------------------
-template <typename T, int VX, int K>
+template <typename T, int VX, int K, int ACC>
 hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
                                  int accumulate, hipStream_t s)
 {
@@ -965,7 +970,7 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
     if (fp.kind == 3 && A.gen == 2) {
         if constexpr (K <= 9 && VX <= 16 / (int)sizeof(T) && VX <= 4) {
 #define DOP_L3E(UNI, W, VY)                                                                  \
-    hipLaunchKernelGGL((k_tiled3e<T, VX, K, UNI, 1, W, VY>), dim3((unsigned)A.nblocks),        \
+    hipLaunchKernelGGL((k_tiled3e<T, VX, K, UNI, 1, W, VY, ACC>), dim3((unsigned)A.nblocks),   \
                        dim3(64 * W), 0, s, A, (T*)du, (const T*)u, accumulate)
 #define DOP_L3EU(W, VY)                                                                      \
     do {                                                                                     \
@@ -987,7 +992,7 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
     const dim3 grid((unsigned)(A.nblocks + nshell));
     if (fp.kind == 2) {
 #define DOP_L2(UNI, PF)                                                                      \
-    hipLaunchKernelGGL((k_tiled2d<T, VX, K, UNI, PF>), grid, dim3(256), 0, s, A, p.dev,        \
+    hipLaunchKernelGGL((k_tiled2d<T, VX, K, UNI, PF, ACC>), grid, dim3(256), 0, s, A, p.dev,   \
                        fp.shell, nshell, (T*)du, (const T*)u, accumulate)
         if (uni) {
             if (A.PF == 4) DOP_L2(true, 4); else DOP_L2(true, 2);
@@ -998,7 +1003,7 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
     } else {
         if constexpr (K <= 9 && VX <= 16 / (int)sizeof(T) && VX <= 4) {
 #define DOP_L3(UNI, W, VY)                                                                   \
-    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, kPF3D<T, K>, W, VY>), grid, dim3(64 * W), 0, s, \
+    hipLaunchKernelGGL((k_tiled3d<T, VX, K, UNI, kPF3D<T, K>, W, VY, ACC>), grid, dim3(64 * W), 0, s, \
                        A, p.dev, fp.shell, nshell, (T*)du, (const T*)u, accumulate)
             constexpr bool w8 = 2 * (K / 2) < 8;   // 8-wave blocks of one row compute 8 - 2H rows
 #define DOP_L3U(W, VY)                                                                       \
@@ -1031,7 +1036,7 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
 }
 
 // The 3-D shell (its own launch, on s2; may be s)
-template <typename T, int K>
+template <typename T, int K, int ACC>
 hipError_t launch_shell3d(const DevicePlan& p, void* du, const void* u, int accumulate, hipStream_t s2)
 {
     const FastPlan& fp = p.fast;
@@ -1040,11 +1045,11 @@ hipError_t launch_shell3d(const DevicePlan& p, void* du, const void* u, int accu
             const dim3 sg((unsigned)fp.strips.nblocks);
             const StripArgs& SA = fp.strips;
             if (SA.vec_ok)
-                hipLaunchKernelGGL((shell::k_shell_strips<T, acc_of<T>(), K, true>), sg,
+                hipLaunchKernelGGL((shell::k_shell_strips<T, ACC, K, true>), sg,
                                    dim3(shell::kShellBlock), 0, s2, p.dev, SA, (T*)du, (const T*)u,
                                    accumulate);
             else
-                hipLaunchKernelGGL((shell::k_shell_strips<T, acc_of<T>(), K, false>), sg,
+                hipLaunchKernelGGL((shell::k_shell_strips<T, ACC, K, false>), sg,
                                    dim3(shell::kShellBlock), 0, s2, p.dev, SA, (T*)du, (const T*)u,
                                    accumulate);
             const hipError_t e = hipGetLastError();
@@ -1055,28 +1060,42 @@ hipError_t launch_shell3d(const DevicePlan& p, void* du, const void* u, int accu
 }
 
 // The tiles on s, then (3-D) the shell on s2: the long bandwidth-bound launch is dispatched first
-template <typename T, int VX, int K>
+template <typename T, int VX, int K, int ACC>
 hipError_t launch_interior(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
                            int accumulate, hipStream_t s, hipStream_t s2)
 {
-    const hipError_t e = launch_interior_tiles<T, VX, K>(p, A, du, u, accumulate, s);
+    const hipError_t e = launch_interior_tiles<T, VX, K, ACC>(p, A, du, u, accumulate, s);
     if (e != hipSuccess || p.fast.kind != 3) return e;
-    return launch_shell3d<T, K>(p, du, u, accumulate, s2);
+    return launch_shell3d<T, K, ACC>(p, du, u, accumulate, s2);
 }
 
+template <typename T, int VX, int ACC>
+hipError_t launch_interior_acc(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
+                               int accumulate, hipStream_t s, hipStream_t s2)
+{
+    switch (A.K) {
+    case 3: return launch_interior<T, VX, 3, ACC>(p, A, du, u, accumulate, s, s2);
+    case 5: return launch_interior<T, VX, 5, ACC>(p, A, du, u, accumulate, s, s2);
+    case 7: return launch_interior<T, VX, 7, ACC>(p, A, du, u, accumulate, s, s2);
+    case 9: return launch_interior<T, VX, 9, ACC>(p, A, du, u, accumulate, s, s2);
+    case 11: return launch_interior<T, VX, 11, ACC>(p, A, du, u, accumulate, s, s2);
+    case 13: return launch_interior<T, VX, 13, ACC>(p, A, du, u, accumulate, s, s2);
+    }
+    return hipErrorInvalidValue;
+}
+
+// the plan's accumulation: the storage's own (f64 / f32) or, for f32 and bf16 storage,
+// f32 values summed in f64 (DOP_ACC_F32_WIDE, what DOP_ACC_AUTO picks for derivative orders
+// above 2)
 template <typename T, int VX>
 hipError_t launch_interior_k(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
                              int accumulate, hipStream_t s, hipStream_t s2)
 {
-    switch (A.K) {
-    case 3: return launch_interior<T, VX, 3>(p, A, du, u, accumulate, s, s2);
-    case 5: return launch_interior<T, VX, 5>(p, A, du, u, accumulate, s, s2);
-    case 7: return launch_interior<T, VX, 7>(p, A, du, u, accumulate, s, s2);
-    case 9: return launch_interior<T, VX, 9>(p, A, du, u, accumulate, s, s2);
-    case 11: return launch_interior<T, VX, 11>(p, A, du, u, accumulate, s, s2);
-    case 13: return launch_interior<T, VX, 13>(p, A, du, u, accumulate, s, s2);
+    if constexpr (!std::is_same<T, double>::value) {
+        if (p.host.accum == kAccF32Wide)
+            return launch_interior_acc<T, VX, kAccF32Wide>(p, A, du, u, accumulate, s, s2);
     }
-    return hipErrorInvalidValue;
+    return launch_interior_acc<T, VX, acc_of<T>()>(p, A, du, u, accumulate, s, s2);
 }
 
 }  // namespace tiled

@@ -148,8 +148,14 @@ int64_t hc_shell(const dop_desc* d, void* du, const void* u, int accumulate)
     using namespace dop;
     switch (P.storage) {
     case kF64: run_shell<double, kAccF64>(P, F.strips, F.args.K, (double*)du, (const double*)u, accumulate); break;
-    case kF32: run_shell<float, kAccF32>(P, F.strips, F.args.K, (float*)du, (const float*)u, accumulate); break;
-    case kBF16: run_shell<bf16, kAccF32>(P, F.strips, F.args.K, (bf16*)du, (const bf16*)u, accumulate); break;
+    case kF32:
+        if (P.accum == kAccF32Wide) run_shell<float, kAccF32Wide>(P, F.strips, F.args.K, (float*)du, (const float*)u, accumulate);
+        else run_shell<float, kAccF32>(P, F.strips, F.args.K, (float*)du, (const float*)u, accumulate);
+        break;
+    case kBF16:
+        if (P.accum == kAccF32Wide) run_shell<bf16, kAccF32Wide>(P, F.strips, F.args.K, (bf16*)du, (const bf16*)u, accumulate);
+        else run_shell<bf16, kAccF32>(P, F.strips, F.args.K, (bf16*)du, (const bf16*)u, accumulate);
+        break;
     }
     return F.shell_total;
 }

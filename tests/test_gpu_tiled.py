@@ -95,6 +95,38 @@ def test_tiled_3d_bitwise(shape, kh, dtype, uniform):
     assert torch.equal(a2, b2)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("shape", [(257, 130), (40, 48, 64), (33, 29, 70), (64, 64, 256)])
+def test_tiled_wide_accumulation_bitwise(shape, dtype):
+    # third derivatives: DOP_ACC_AUTO sums f32 values in f64 (DOP_ACC_F32_WIDE); the tiles
+    # and the 3-D shell run that arithmetic and must match the generic kernel bit for bit
+    rng = np.random.default_rng(21)
+    jsize = tuple(reversed(shape))
+    ops = []
+    for N in range(1, len(shape) + 1):
+        n = jsize[N - 1]
+        dx = (1.0 / (n + 1)) * (1 + 0.5 * rng.random(n + 1))
+        ops += [D.CenteredDifference[N](3, 4, dx, n), D.CenteredDifference[N](2, 2, dx, n)]
+    A = ops[0]
+    for L in ops[1:]:
+        A = A + L
+    Q = D.compose(*D.RobinBC((1.0, 0.5, 0.1), (1.0, 0.5, -0.2), tuple(1.0 / (s + 1) for s in jsize), 1, jsize))
+    A = A * Q
+    u = torch.from_numpy(rng.uniform(-1, 1, shape)).to("cuda").to(dtype)
+    print(f"case: wide {shape} {dtype}", flush=True)
+    tiled = D.plans_for(A, u)[0]
+    gen = generic_of(A, u)
+    a = torch.empty_like(u)
+    b = torch.empty_like(u)
+    tiled.apply(a, u)
+    gen.apply(b, u)
+    assert torch.equal(a, b)
+    a2, b2 = a.clone(), b.clone()
+    tiled.apply(a2, u, accumulate=True)
+    gen.apply(b2, u, accumulate=True)
+    assert torch.equal(a2, b2)
+
+
 def test_autotune_candidates_agree():
     A, u = build((512, 512), 2, torch.float64, seed=3)
     p = D.plans_for(A, u)[0]

@@ -118,6 +118,29 @@ def test_shell_strips_equal_generic(hc, shape, kh, storage, uniform):
         assert pts[3] < pts[:3].sum(), pts
 
 
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("shape", [(20, 24, 64), (17, 33, 36)])
+def test_shell_strips_wide_accumulation(hc, storage, shape):
+    # third derivatives: DOP_ACC_AUTO sums f32 values in f64 (DOP_ACC_F32_WIDE), which the
+    # tiled path and its shell now run too
+    rng = np.random.default_rng(11)
+    jsize = tuple(reversed(shape))
+    ops = []
+    for N in range(1, 4):
+        n = jsize[N - 1]
+        dx = (1.0 / (n + 1)) * (1 + 0.5 * rng.random(n + 1))
+        ops += [D.CenteredDifference[N](3, 4, dx, n), D.CenteredDifference[N](2, 2, dx, n)]
+    Q = D.compose(*D.RobinBC((1.0, 0.5, 0.1), (1.0, 0.5, -0.2), tuple(1.0 / (s + 1) for s in jsize),
+                             1, jsize))
+    bcs = _bcs_of(Q, 3, jsize)[0]
+    d, keep = make_desc(ops, jsize, STORAGE[storage][0], input_kind=_abi.INPUT_GHOST,
+                        semantics=_abi.SEM_LINES, bcs=bcs)
+    assert d.accum == _abi.ACC_AUTO
+    u = values(shape, storage, rng)
+    check(hc, d, u, storage, 0)
+    check(hc, d, u, storage, 1)
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 @pytest.mark.parametrize("periodic", [0.0, 0.3])
 def test_shell_strips_per_line_bcs(hc, storage, periodic):
