@@ -260,9 +260,10 @@ void build_fast(const HostPlan& H, FastPlan* fp)
 }
 
 // Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap x
-// points per lane {widest 16-byte, twice that}; 3-D: chunk depth {64,32} x block shape
-// (waves, rows per wave) {(8,1), (16,1), (8,2), (16,2), (8,3)} x XCD remap x points per lane
-// {widest, half}.  A block stages waves * rows y rows and computes all but the 2H halo rows.
+// points per lane {widest 16-byte, twice that} x marches {all downwards, alternating}; 3-D:
+// chunk depth {64,32} x block shape (waves, rows per wave) {(8,1), (16,1), (8,2), (16,2),
+// (8,3)} x XCD remap x points per lane {widest, half}, then the edge-loaded forms.  A block
+// stages waves * rows y rows and computes all but the 2H halo rows.
 namespace {
 bool wide_ok(const FastPlan& fp) { return fp.VX > 1 && fp.args.n[0] % (2 * fp.VX) == 0; }
 
@@ -273,7 +274,7 @@ int gen2_count(const FastPlan& fp) { return fp.VX > 1 ? 32 : 16; }
 int fast_variant_count(const FastPlan& fp)
 {
     if (fp.kind == 1) return 1;
-    if (fp.kind == 2) return wide_ok(fp) ? 32 : 16;
+    if (fp.kind == 2) return wide_ok(fp) ? 64 : 32;
     if (fp.kind == 3) return gen1_count(fp) + gen2_count(fp);
     return 1;
 }
@@ -309,7 +310,10 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         static const int chunks[4] = {32, 16, 64, 8};
         A.chunk = chunks[(v >> 1) & 3];
         A.PF = ((v >> 3) & 1) ? 4 : 2;
-        A.VX = ((v >> 4) & 1) ? 2 * fp.VX : fp.VX;
+        // bit 4: twice the points per lane (when rows allow), next bit: alternating marches
+        const bool wide = wide_ok(fp);
+        A.VX = (wide && ((v >> 4) & 1)) ? 2 * fp.VX : fp.VX;
+        A.alt = (v >> (wide ? 5 : 4)) & 1;
         A.W = 4;
         A.VY = 1;
     } else {

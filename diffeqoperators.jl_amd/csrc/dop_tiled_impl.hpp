@@ -411,43 +411,57 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     }
 
     const T* ucol = u + xl;
-    C q[R][VX];
-    // rows past the chunk's last tap row (the prefetch of its last steps) reload that row,
-    // already in cache, instead of reading rows this wave never uses
+    // rows outside the chunk's tap rows (the prefetch of its last steps) reload its last tap
+    // row, already in cache, instead of reading rows this wave never uses
+    const int ylo = max(y0 - H, 0);
     const int ylast = min(y1 + H - 1, n1 - 1);
     auto load_row = [&](int row, C* dst) {
-        row = min(max(row, 0), ylast);
+        row = min(max(row, ylo), ylast);
         VecIO<T, VX>::ld(ucol + (int64_t)row * s1, dst);
     };
+    const int nrows = y1 - y0;
+    // The march over the chunk's rows, downwards (DIR = 1) or upwards (DIR = -1).  Ring slot
+    // of a row: (its distance from the first output row along the march + H) mod R, so the
+    // slots of a step are compile-time; the taps are summed in ascending row order either way.
+    auto march = [&](auto dirc) {
+        constexpr int DIR = decltype(dirc)::value;
+        const int ystart = DIR > 0 ? y0 : y1 - 1;
+        C q[R][VX];
 #pragma unroll
-    for (int s = 0; s < R - 1; ++s) load_row(y0 - H + s, q[s]);
-
-    for (int yb = y0; yb < y1; yb += R) {
-        auto step = [&](auto jc) -> bool {
-            constexpr int j = decltype(jc)::value;
-            const int y = yb + j;
-            if (y >= y1) return false;
-            load_row(y + H + PF, q[(j + K - 1 + PF) % R]);
-            C xw[NWIN];
-            x_window_dpp<T, VX, H>(q[(j + H) % R], xw);
-            C wy[K];
+        for (int s = 0; s < R - 1; ++s) load_row(ystart + DIR * (s - H), q[s]);
+        for (int t0 = 0; t0 < nrows; t0 += R) {
+            auto step = [&](auto jc) -> bool {
+                constexpr int j = decltype(jc)::value;
+                const int t = t0 + j;
+                if (t >= nrows) return false;
+                const int y = ystart + DIR * t;
+                load_row(y + DIR * (H + PF), q[(j + K - 1 + PF) % R]);
+                C xw[NWIN];
+                x_window_dpp<T, VX, H>(q[(j + H) % R], xw);
+                C wy[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) wy[k] = UNI ? wyu[k] : wyt[(int64_t)y * K + k];
-            Sum<C, ACC> out[VX];
+                for (int k = 0; k < K; ++k) wy[k] = UNI ? wyu[k] : wyt[(int64_t)y * K + k];
+                Sum<C, ACC> out[VX];
 #pragma unroll
-            for (int v = 0; v < VX; ++v) {
-                Sum<C, ACC> s = 0;
+                for (int v = 0; v < VX; ++v) {
+                    Sum<C, ACC> s = 0;
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fsum<C, ACC>(wx[v][k], xw[v + k], s);
+                    for (int k = 0; k < K; ++k) s = fsum<C, ACC>(wx[v][k], xw[v + k], s);
 #pragma unroll
-                for (int k = 0; k < K; ++k) s = fsum<C, ACC>(wy[k], q[(j + k) % R][v], s);
-                out[v] = s;
-            }
-            if (st_ok) store_sums<T, VX, ACC>(du + (int64_t)y * s1 + xs, out, accumulate);
-            return true;
-        };
-        if (!ring_steps<0, R>(step)) break;
-    }
+                    for (int k = 0; k < K; ++k)
+                        s = fsum<C, ACC>(wy[k], q[(DIR > 0 ? j + k : j + 2 * H - k) % R][v], s);
+                    out[v] = s;
+                }
+                if (st_ok) store_sums<T, VX, ACC>(du + (int64_t)y * s1 + xs, out, accumulate);
+                return true;
+            };
+            if (!ring_steps<0, R>(step)) break;
+        }
+    };
+    // alternate chunks march towards each other's shared edge: both read the halo rows there at
+    // the same time, and neighbouring chunks on one XCD (the remapped order) share them in L2
+    if (A.alt && (ch & 1)) march(std::integral_constant<int, -1>{});
+    else march(std::integral_constant<int, 1>{});
 }
 
 // ---------------------------------------------------------------------------------------

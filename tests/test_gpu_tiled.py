@@ -154,7 +154,8 @@ def n_variants(shape, dtype):
     vx = 2 if dtype == torch.float64 else 4
     vx = vx if n0 % vx == 0 else 1
     if len(shape) == 2:
-        return 32 if vx > 1 and n0 % (2 * vx) == 0 else 16
+        # chunk x prefetch x remap (16) x points per lane (2, when rows allow) x marches (2)
+        return 64 if vx > 1 and n0 % (2 * vx) == 0 else 32
     # overlapping segments (40 / 20) and edge-loaded segments (32 / 16)
     return 72 if vx > 1 else 36
 
