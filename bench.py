@@ -12,6 +12,7 @@ Other workloads are reported in the same line under "also" (`--no-also` skips th
 selected with `--config laplacian2d|batched|served`.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -32,20 +33,30 @@ def _sync():
 
 
 def time_applies(fn, steps, warmup):
-    """Wall time and HIP-event time (on torch's current stream, where libdop launches)."""
+    """Wall time and HIP-event time (on torch's current stream, where libdop launches).  The
+    garbage collector is off inside the timed loop (as in timeit): a full collection stalls the
+    host for tens of milliseconds, long enough to drain the launch queue and leave the GPU idle
+    inside the event span."""
     import torch
     for _ in range(warmup):
         fn()
     _sync()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(s)
-    for _ in range(steps):
-        fn()
-    e1.record(s)
-    _sync()
-    wall = time.perf_counter() - t0
+    was = gc.isenabled()
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        e0.record(s)
+        for _ in range(steps):
+            fn()
+        e1.record(s)
+        _sync()
+        wall = time.perf_counter() - t0
+    finally:
+        if was:
+            gc.enable()
     return wall, e0.elapsed_time(e1) / 1e3
 
 

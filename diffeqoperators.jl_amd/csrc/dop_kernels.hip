@@ -6,6 +6,8 @@
 // operator, composite_operators.jl:64-65).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dop_eval.hpp"
 #include "dop_kernels.hpp"
 
@@ -100,17 +102,227 @@ __global__ __launch_bounds__(kBlock) void k_apply_line(const PlanDev* __restrict
     }
 }
 
+// Four consecutive points of one row per thread (rows of n0 % 4 == 0 points): the same
+// arithmetic as k_apply_line, with the values of the four points' taps read as 4-value vector
+// loads (element-aligned; the padded inputs of the vector calculus have odd strides), uniform
+// weights read once per thread, and one vector store -- a quarter of the memory instructions,
+// which bound the one-point form.
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+    typedef float V __attribute__((ext_vector_type(4), aligned(4)));
+    __device__ static void ld(const float* p, float* o)
+    {
+        const V v = *reinterpret_cast<const V*>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    }
+    __device__ static void st(float* p, const double* v)
+    {
+        V o;
+        o.x = (float)v[0]; o.y = (float)v[1]; o.z = (float)v[2]; o.w = (float)v[3];
+        *reinterpret_cast<V*>(p) = o;
+    }
+};
+template <>
+struct Quad<double> {
+    typedef double V __attribute__((ext_vector_type(2), aligned(8)));
+    __device__ static void ld(const double* p, double* o)
+    {
+        const V a = *reinterpret_cast<const V*>(p);
+        const V b = *reinterpret_cast<const V*>(p + 2);
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    }
+    __device__ static void st(double* p, const double* v)
+    {
+        V a, b;
+        a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3];
+        *reinterpret_cast<V*>(p) = a;
+        *reinterpret_cast<V*>(p + 2) = b;
+    }
+};
+
+template <typename T, int ACC, int K>
+__global__ __launch_bounds__(kBlock) void k_apply_line4(const PlanDev* __restrict__ Pp,
+                                                       T* __restrict__ du,
+                                                       const T* __restrict__ src, int accumulate,
+                                                       int a)
+{
+    using W = typename Acc<ACC>::W;
+    const PlanDev& P = *Pp;
+    const AxisDev& X = P.ax[a];
+    const uint32_t g0 = (uint32_t)P.n[0] >> 2, n0 = (uint32_t)P.n[0], n1 = (uint32_t)P.n[1];
+    const uint32_t rows = (uint32_t)(P.total / P.n[0]);
+    const int64_t S = P.ssrc[a];
+    const W* wt = Acc<ACC>::table(X);
+    const int64_t ws = X.uniform ? 1 : X.n;
+    const int jd = X.dmin + (P.ghost ? 0 : 1);
+    const int64_t tsh = P.tshift[a];
+    int64_t sst[kMaxDim];
+#pragma unroll
+    for (int b = 0; b < kMaxDim; ++b) sst[b] = (b != a && b < P.ndim) ? P.ssrc[b] : 0;
+    const int rlo = X.rlo, rhi = X.rhi;
+    const bool uni = X.uniform != 0;
+    // uniform weights: read once
+    W wu[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) wu[k] = uni ? tab(wt, k) : W(0);
+    // groups of four points along axis 0: on the active axis 0 the groups [gA, gB) of a row
+    // hold interior rows only; the points of the row's other groups (boundary rows and their
+    // neighbours) are evaluated one per thread afterwards, so that no wave serialises four
+    // boundary rows per thread
+    const uint32_t gA = a == 0 ? (uint32_t)(rlo + 3) / 4 : 0;
+    const uint32_t gB = a == 0 ? max(gA, (uint32_t)rhi / 4) : g0;
+    const uint32_t gin = gB - gA;
+    const uint32_t nq = rows * gin;
+    const uint32_t nb = gridDim.x, per = nb >> 3;
+    const uint32_t bx = blockIdx.x < per * 8 ? (blockIdx.x & 7) * per + (blockIdx.x >> 3) : blockIdx.x;
+    const uint32_t span = ((nq + nb - 1) / nb + kBlock - 1) / kBlock * kBlock;
+    const uint32_t gend = min(nq, (uint32_t)min((uint64_t)(bx + 1) * span, (uint64_t)nq));
+    for (uint32_t g = bx * span + threadIdx.x; g < gend; g += kBlock) {
+        const uint32_t row = g / gin;
+        const uint32_t gi = gA + (g - row * gin);
+        const uint32_t q2 = row / n1;
+        int64_t idx[3];
+        idx[0] = (int64_t)gi * 4;
+        idx[1] = row - q2 * n1;
+        idx[2] = q2;
+        const int64_t base = tsh + idx[0] * sst[0] + idx[1] * sst[1] + idx[2] * sst[2];
+        Acc<ACC> acc[4];
+        if (a != 0) {
+            // the four points share row r of the axis
+            const int r = (int)idx[a];
+            if (r >= rlo && r < rhi) {
+                const int64_t p0 = base + (int64_t)(r + jd) * S;
+                W wv[K];
+                T xv[K][4];
+                if (uni) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) wv[k] = wu[k];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) wv[k] = tab(wt, k * ws + r);
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) Quad<T>::ld(src + p0 + k * S, xv[k]);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i].add(wv[k], (W)xv[k][i]);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    int64_t id[3] = {idx[0] + i, idx[1], idx[2]};
+                    acc[i].add_sum(boundary_row<T>(P, X, src, r, base + i, S, line_of(X, id)));
+                }
+            }
+        } else {
+            // rows r0 .. r0 + 3, all interior: the window of K + 3 values read as groups of
+            // four, the last group ending at the window's end (no value past it is read)
+            const int r0 = (int)idx[0];
+            constexpr int NW = K + 3;
+            constexpr int NG = (NW + 3) / 4;
+            T xv[NW];
+#pragma unroll
+            for (int m = 0; m < NG; ++m) {
+                const int o = 4 * m < NW - 4 ? 4 * m : NW - 4;
+                T t[4];
+                Quad<T>::ld(src + base + (int64_t)(r0 + jd) + o, t);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (o + e >= 4 * m) xv[o + e] = t[e];
+            }
+            W wv[K][4];
+            if (uni) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) wv[k][i] = wu[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) wv[k][i] = tab(wt, k * ws + r0 + i);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i].add(wv[k][i], (W)xv[i + k]);
+            }
+        }
+        const int64_t p = (int64_t)row * n0 + idx[0];
+        double v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = acc[i].value();
+        if (accumulate) {
+            T old[4];
+            Quad<T>::ld(du + p, old);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] += (double)old[i];
+        }
+        Quad<T>::st(du + p, v);
+    }
+    if (a != 0) return;
+    // the points of each row's edge groups, one per thread (k_apply_line's arithmetic)
+    const uint32_t ne = 4 * (g0 - gin);
+    const uint32_t nE = rows * ne;
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < nE; e += nb * kBlock) {
+        const uint32_t row = e / ne;
+        const uint32_t j = e - row * ne;
+        const uint32_t x = j < 4 * gA ? j : 4 * gB + (j - 4 * gA);
+        const uint32_t q2 = row / n1;
+        int64_t idx[3];
+        idx[0] = x;
+        idx[1] = row - q2 * n1;
+        idx[2] = q2;
+        const int64_t base = tsh + idx[1] * sst[1] + idx[2] * sst[2];
+        const int r = (int)x;
+        Acc<ACC> acc;
+        if (r >= rlo && r < rhi) {
+            const int64_t p0 = base + (int64_t)(r + jd);
+            W wv[K], xv[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                wv[k] = uni ? wu[k] : tab(wt, k * ws + r);
+                xv[k] = (W)Store<T>::ld(src, p0 + k);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc.add(wv[k], xv[k]);
+        } else {
+            acc.add_sum(boundary_row<T>(P, X, src, r, base, 1, line_of(X, idx)));
+        }
+        const int64_t p = (int64_t)row * n0 + x;
+        double v = acc.value();
+        if (accumulate) v += (double)Store<T>::ld(du, p);
+        Store<T>::st(du, p, v);
+    }
+}
+
 template <typename T, int ACC>
 static hipError_t launch_line(const PlanDev& host, const PlanDev* dev, void* du, const void* u,
                               int accumulate, int a, hipStream_t s)
 {
-    // enough resident blocks for every CU several times over, each with a long span
-    int64_t blocks = (host.total + kBlock - 1) / kBlock;
+    // enough resident blocks for every CU several times over, each with a long span; four
+    // points per thread when rows hold whole groups of four (f32 and f64 storage)
+    const bool quad = !std::is_same<T, bf16>::value && host.n[0] % 4 == 0 && host.ssrc[0] == 1;
+    int64_t blocks = ((quad ? host.total / 4 : host.total) + kBlock - 1) / kBlock;
     if (blocks > kLineBlocks) blocks = kLineBlocks;
     if (blocks < 1) blocks = 1;
 #define DOP_LINE(KK)                                                                         \
-    hipLaunchKernelGGL((k_apply_line<T, ACC, KK>), dim3((unsigned)blocks), dim3(kBlock), 0, s, \
-                       dev, (T*)du, (const T*)u, accumulate, a)
+    do {                                                                                     \
+        if constexpr (!std::is_same<T, bf16>::value) {                                       \
+            if (quad) {                                                                      \
+                hipLaunchKernelGGL((k_apply_line4<T, ACC, KK>), dim3((unsigned)blocks),       \
+                                   dim3(kBlock), 0, s, dev, (T*)du, (const T*)u, accumulate, a); \
+                break;                                                                       \
+            }                                                                                \
+        }                                                                                    \
+        hipLaunchKernelGGL((k_apply_line<T, ACC, KK>), dim3((unsigned)blocks), dim3(kBlock), 0, s, \
+                           dev, (T*)du, (const T*)u, accumulate, a);                         \
+    } while (0)
     switch (host.ax[a].K) {
     case 2: DOP_LINE(2); break;
     case 3: DOP_LINE(3); break;

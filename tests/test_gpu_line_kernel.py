@@ -87,3 +87,35 @@ def test_padded_plans_bitwise(rng, dtype):
         outs.append((g, dv, c, p, q, nl))
     for x, y in zip(outs[0], outs[1]):
         assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n", [4, 8, 12, 16, 20, 3000])
+@pytest.mark.parametrize("d,approx", [(1, 2), (2, 4), (4, 6), (2, 8)])
+@pytest.mark.parametrize("uniform", [True, False])
+def test_line_kernel_groups_of_four_bitwise(rng, dtype, n, d, approx, uniform):
+    """Rows of a multiple of four points take the four-point form: boundary rows at both ends,
+    interior rows inside the edge groups, rows whose interior holds no whole group."""
+    if d + approx > n:
+        pytest.skip("the boundary stencil is longer than the grid")
+    print(f"case: line4 n={n} d={d} a={approx} uniform={uniform} {dtype}", flush=True)
+    dx = 1.0 / (n + 1) if uniform else (1 + 0.5 * rng.random(n + 1)) / (n + 1)
+    L = D.CenteredDifference(d, approx, dx, n)
+    A = L * D.RobinBC((1.0, 0.5, 0.3), (1.0, 0.4, -0.2), 1.0 / (n + 1), 2)
+    u = torch.from_numpy(rng.uniform(-1, 1, n)).to("cuda").to(dtype)
+    _check(A, u)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_line_kernel_groups_of_four_3d_bitwise(rng, dtype, axis):
+    """A 3-D array with 44 points per row: the active axis is the row axis (windows read in
+    groups of four) or a strided axis (four points share one tap row)."""
+    shape = (12, 9, 44)
+    jsize = tuple(reversed(shape))
+    m = jsize[axis - 1]
+    print(f"case: line4 3d axis {axis} {dtype}", flush=True)
+    L = D.CenteredDifference[axis](2, 4, 0.05 * (1 + 0.5 * rng.random(m + 1)), m)
+    A = L * D.compose(*D.Dirichlet0BC(float, jsize))
+    u = torch.from_numpy(rng.uniform(-1, 1, shape)).to("cuda").to(dtype)
+    _check(A, u)
