@@ -34,4 +34,13 @@ int apply_variants(const DevicePlan& p);
 // *dst = v on stream s (the new values of a plan, carried by the launch itself).
 hipError_t launch_set_plan(PlanDev* dst, const PlanDev& v, hipStream_t s);
 
+// The library's own streams for side work on the current device (non-blocking, default
+// priority), created on first use and kept for the life of the process: the shell of a 3-D
+// apply runs on stream 0, the server's size classes on streams 0, 1, 2.  HIP maps a process's
+// streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default) and streams that share a queue
+// run in order; with the caller's stream, three streams of the library's own fill four queues,
+// where a stream per user (the shell's and each server's) would make two of them share one.
+constexpr int kSideStreams = 3;
+hipError_t side_stream(int i, hipStream_t* out);
+
 }  // namespace dop

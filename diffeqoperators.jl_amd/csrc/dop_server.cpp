@@ -6,7 +6,8 @@
 //             fits in the device arena and fewer than max_active requests are resident;
 //             its initial field is copied in asynchronously
 //   step      resident requests are split into size classes by grid points; each class has
-//             its own stream and advances its requests one RK4 step per iteration (four
+//             its own stream (the library's side stream of that index, so the classes and the
+//             caller's stream fill HIP's four hardware queues without sharing) and advances its requests one RK4 step per iteration (four
 //             grouped launches over the class, the work list rebuilt only when the class
 //             changes).  The classes run beside each other on the GPU, so a small request
 //             steps at the pace of its own class instead of the largest resident grid's.
@@ -175,7 +176,6 @@ struct dop_server {
                 if (s.used) (void)hipEventDestroy(s.used);
             }
             for (auto& e : c.step_events) (void)hipEventDestroy(e);
-            if (c.st) (void)hipStreamDestroy(c.st);
         }
         if (arena_base) (void)hipFree(arena_base);
     }
@@ -309,8 +309,10 @@ int dop_server_create(int64_t hbm_budget_bytes, int max_active, int chunk, void*
         return dop::set_error(DOP_ENOMEM, "server: arena allocation failed");
     }
     s->arena.init(s->arena_base, (size_t)s->budget);
-    for (auto& c : s->cls) {
-        bool ok = hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking) == hipSuccess;
+    static_assert(kClasses <= dop::kSideStreams, "a side stream per class");
+    for (int k = 0; k < kClasses; ++k) {
+        Class& c = s->cls[k];
+        bool ok = dop::side_stream(k, &c.st) == hipSuccess;
         for (auto& sl : c.slots)
             ok = ok && hipEventCreateWithFlags(&sl.used, hipEventDisableTiming) == hipSuccess;
         c.step_events.resize(kMaxInflight + 1, nullptr);

@@ -2,6 +2,8 @@
 // (kernels: dop_tiled_impl.hpp, instantiated in dop_tiled_{f64,f32,bf16}_v*.hip).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "dop_kernels.hpp"
 
 namespace dop {
@@ -29,11 +31,31 @@ hipError_t launch_tiled_bf16_v4(const DevicePlan& p, const FastArgs& A, void* du
 hipError_t launch_tiled_bf16_v8(const DevicePlan& p, const FastArgs& A, void* du, const void* u,
                              int accumulate, hipStream_t s, hipStream_t s2);
 
+hipError_t side_stream(int i, hipStream_t* out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static hipStream_t pool[kMaxDevices][kSideStreams] = {};
+    if (i < 0 || i >= kSideStreams || !out) return hipErrorInvalidValue;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!pool[dev][i] && (e = hipStreamCreateWithFlags(&pool[dev][i], hipStreamNonBlocking)) != hipSuccess) {
+        pool[dev][i] = nullptr;
+        return e;
+    }
+    *out = pool[dev][i];
+    return hipSuccess;
+}
+
 namespace {
 
 // The second stream of a 3-D apply: the shell (latency-bound, few points) runs there beside the
-// tiles (bandwidth-bound); they write disjoint points.  One stream and two events per thread and
-// device, created on first use and kept for the life of the process (default priority).
+// tiles (bandwidth-bound); they write disjoint points.  The stream is the library's side stream
+// 0; the fork and join events are per thread and device (created on first use and kept), so
+// threads that apply at once order their own shells correctly on the shared stream.
 struct ShellStream {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
@@ -49,9 +71,9 @@ hipError_t shell_stream(ShellStream** out)
     if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
     ShellStream& S = per[dev];
     if (!S.s) {
-        if ((e = hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking)) != hipSuccess) return e;
         if ((e = hipEventCreateWithFlags(&S.fork, hipEventDisableTiming)) != hipSuccess) return e;
         if ((e = hipEventCreateWithFlags(&S.join, hipEventDisableTiming)) != hipSuccess) return e;
+        if ((e = side_stream(0, &S.s)) != hipSuccess) return e;
     }
     *out = &S;
     return hipSuccess;

@@ -240,8 +240,10 @@ void dop_rk4_destroy(dop_rk4* rk);
  * is not open (dop_server_set_open(srv, 1) keeps it waiting for submissions until it is set
  * to 0).  A request whose state alone exceeds the budget is refused at submission
  * (DOP_EBUDGET); on an error the run stops and the resident requests' memory is returned.
- * `stream` is unused (the classes run on the server's own streams, synchronized before
- * dop_server_run returns); dop_server_times / dop_server_stats describe the last run and are
+ * `stream` is unused (the classes run on the library's three side streams of the device, the
+ * ones the second half of a 3-D apply also uses, synchronized before dop_server_run returns:
+ * with the caller's stream they fill HIP's default four hardware queues, so no class waits
+ * behind another; servers run at once share them and run in turn); dop_server_times / dop_server_stats describe the last run and are
  * read after it. */
 typedef struct dop_server dop_server;
 int dop_server_create(int64_t hbm_budget_bytes, int max_active, int chunk, void* stream,
