@@ -260,12 +260,16 @@ void build_fast(const HostPlan& H, FastPlan* fp)
 }
 
 // Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap x
-// points per lane {widest 16-byte, twice that} x marches {all downwards, alternating}; 3-D:
+// points per lane {widest 16-byte, twice that} x marches {all downwards, alternating} x
+// segments {overlapping, edge-loaded (when the x half-width is at most the points per lane)}; 3-D:
 // chunk depth {64,32} x block shape (waves, rows per wave) {(8,1), (16,1), (8,2), (16,2),
 // (8,3)} x XCD remap x points per lane {widest, half}, then the edge-loaded forms.  A block
 // stages waves * rows y rows and computes all but the 2H halo rows.
 namespace {
 bool wide_ok(const FastPlan& fp) { return fp.VX > 1 && fp.args.n[0] % (2 * fp.VX) == 0; }
+// 2-D edge-loaded segments: one edge group per side at the narrowest points per lane
+bool edge2_ok(const FastPlan& fp) { return fp.args.K / 2 <= fp.VX; }
+int base2_count(const FastPlan& fp) { return wide_ok(fp) ? 64 : 32; }
 
 int gen1_count(const FastPlan& fp) { return fp.VX > 1 ? 40 : 20; }
 int gen2_count(const FastPlan& fp) { return fp.VX > 1 ? 32 : 16; }
@@ -274,7 +278,7 @@ int gen2_count(const FastPlan& fp) { return fp.VX > 1 ? 32 : 16; }
 int fast_variant_count(const FastPlan& fp)
 {
     if (fp.kind == 1) return 1;
-    if (fp.kind == 2) return wide_ok(fp) ? 64 : 32;
+    if (fp.kind == 2) return base2_count(fp) * (edge2_ok(fp) ? 2 : 1);
     if (fp.kind == 3) return gen1_count(fp) + gen2_count(fp);
     return 1;
 }
@@ -305,6 +309,10 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         *out = A;
         return true;
     }
+    if (fp.kind == 2 && v >= base2_count(fp)) {
+        A.gen = 2;
+        v -= base2_count(fp);
+    }
     A.xcd_remap = (v & 1) ? 0 : 1;
     if (fp.kind == 2) {
         static const int chunks[4] = {32, 16, 64, 8};
@@ -327,7 +335,12 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
         if (A.VY == 1 && 2 * (A.K / 2) >= 8) A.W = 16;   // a block must compute some rows
         A.VX = ((v >> 2) / 5) ? fp.VX / 2 : fp.VX;
     }
-    {
+    if (A.gen == 2) {
+        // segments of 64*VX points at multiples of 64*VX, not overlapping
+        const int seg = 64 * A.VX;
+        A.xbase = A.lo[0] / seg * seg;
+        A.nxseg = (int)ceil_div(A.hi[0] - A.xbase, seg);
+    } else {
         // segments of 64*VX points overlapping by 2*PADL (the x halo, whole lanes)
         const int H = A.K / 2;
         const int padl = (H + A.VX - 1) / A.VX * A.VX;

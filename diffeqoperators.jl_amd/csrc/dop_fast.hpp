@@ -85,8 +85,8 @@ struct FastArgs {
     int32_t gx;            // 2-D: blocks across x (4 segments each)
     int32_t nblocks;
     int32_t xcd_remap;     // keep neighbouring tiles on one XCD
-    int32_t gen;           // 3-D: 1 overlapping segments (x halo from the segment's own end
-                           // lanes), 2 edge-loaded segments (k_tiled3e)
+    int32_t gen;           // 1 overlapping segments (x halo from the segment's own end lanes),
+                           // 2 edge-loaded segments (3-D: k_tiled3e; 2-D: k_tiled2d's EDG form)
     int32_t alt;           // 2-D: odd row chunks march upwards (towards their neighbours)
     int32_t uniform[3];    // weights identical for every row of the axis
     const double* w64[3];  // [uniform ? 1 : n][K]

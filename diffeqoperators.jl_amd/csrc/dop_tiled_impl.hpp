@@ -351,14 +351,85 @@ __device__ inline void x_window_dpp(const typename Store<T>::C* own, typename St
     }
 }
 
+// lane i <- lane i-1 (lane 0 keeps `fill`) / lane i <- lane i+1 (lane 63 keeps `fill`)
+__device__ inline float dpp_shr1_fill(float fill, float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                                  __builtin_bit_cast(int, v),
+                                                                  0x138, 0xf, 0xf, false));
+}
+__device__ inline float dpp_shl1_fill(float fill, float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                                  __builtin_bit_cast(int, v),
+                                                                  0x130, 0xf, 0xf, false));
+}
+__device__ inline double dpp2(double fill, double v, int ctrl_is_shl)
+{
+    const uint64_t f = __builtin_bit_cast(uint64_t, fill), b = __builtin_bit_cast(uint64_t, v);
+    int lo, hi;
+    if (ctrl_is_shl) {
+        lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, 0x130, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), 0x130,
+                                         0xf, 0xf, false);
+    } else {
+        lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, 0x138, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), 0x138,
+                                         0xf, 0xf, false);
+    }
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ inline double dpp_shr1_fill(double fill, double v) { return dpp2(fill, v, 0); }
+__device__ inline double dpp_shl1_fill(double fill, double v) { return dpp2(fill, v, 1); }
+
+// x window xw[e] = u(xs - H + e), e < VX + 2H, of a lane's VX points: its own values, its
+// neighbours' through DPP, and at the wave's ends the edge groups eg[g] (lane 0: group
+// -(g+1) of the segment, lane 63: group 64+g).
+template <typename C, int VX, int H>
+__device__ inline void x_window_edges(const C* own, const C (*eg)[VX], C* xw)
+{
+    constexpr int NE = (H + VX - 1) / VX;
+#pragma unroll
+    for (int v = 0; v < VX; ++v) xw[H + v] = own[v];
+    C l[VX], r[VX];
+#pragma unroll
+    for (int v = 0; v < VX; ++v) {
+        l[v] = own[v];
+        r[v] = own[v];
+    }
+#pragma unroll
+    for (int d = 1; d <= NE; ++d) {
+#pragma unroll
+        for (int v = 0; v < VX; ++v) {
+            const int el = H - d * VX + v;              // u(xs - d*VX + v)
+            const int er = H + VX + (d - 1) * VX + v;   // u(xs + d*VX + v)
+            // value v of the group at distance d lands in the window (and then so does every
+            // nearer one, so the shift chain of v runs up to the farthest such d)
+            if (el >= 0) {
+                l[v] = dpp_shr1_fill(eg[d - 1][v], l[v]);
+                xw[el] = l[v];
+            }
+            if (er < VX + 2 * H) {
+                r[v] = dpp_shl1_fill(eg[d - 1][v], r[v]);
+                xw[er] = r[v];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // 2-D: a wave loads an aligned 64*VX-point row segment and marches a chunk of rows with a
-// register ring of K + PF rows (PF rows prefetched ahead).  The segment's first and last
-// PADL points (whole lanes) are the x halo: segments overlap by 2*PADL, the x window comes
-// from neighbouring lanes through DPP, and only the middle lanes store.  Loads are clamped
-// into the array and issued on every path, stores are one predicated vector store, so the
-// in-order load counter waits stay precise and the ring stays in flight.
-template <typename T, int VX, int K, bool UNI, int PF, int ACC>
+// register ring of K + PF rows (PF rows prefetched ahead).  Overlapping form (EDG false): the
+// segment's first and last PADL points (whole lanes) are the x halo, segments overlap by
+// 2*PADL, the x window comes from neighbouring lanes through DPP, and only the middle lanes
+// store.  Edge-loaded form (EDG true, H <= VX): segments start at multiples of 64*VX points
+// and do not overlap (every row read is whole 128-byte lines and no point is read twice by
+// neighbouring segments); each lane also loads one edge group per row -- lanes 0-31 the
+// group left of the segment, lanes 32-63 the one right of it, two addresses per wave -- and
+// lanes 0 and 63 feed theirs into the DPP window.  Loads are clamped into the array and issued
+// on every path, stores are one predicated vector store, so the in-order load counter waits
+// stay precise and the ring stays in flight.
+template <typename T, int VX, int K, bool UNI, int PF, int ACC, bool EDG = false>
 __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev* __restrict__ Pp,
                                                 const ShellArgs SH, int nshell,
                                                 T* __restrict__ du, const T* __restrict__ u,
@@ -388,11 +459,15 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     const int64_t n0 = A.n[0];
     const int n1 = (int)A.n[1];
     const int64_t s1 = A.s1;
-    const int64_t xs = (int64_t)A.xbase + (int64_t)xseg * (64 * VX - 2 * PADL) + lane * VX;
+    static_assert(!EDG || H <= VX, "one edge group per side");
+    const int64_t xs0 = (int64_t)A.xbase + (int64_t)xseg * (EDG ? 64 * VX : 64 * VX - 2 * PADL);
+    const int64_t xs = xs0 + lane * VX;
     // A lane whose group lies outside [0, n0) loads a clamped (other) group: only outputs in
     // the box are stored and their windows stay within H of the box, inside the array.
     const int64_t xl = min(max(xs, (int64_t)0), n0 - VX);
-    const bool st_ok = lane >= LEDGE && lane < 64 - LEDGE && xs >= A.lo[0] && xs < A.hi[0];
+    const bool st_ok = (EDG || (lane >= LEDGE && lane < 64 - LEDGE)) && xs >= A.lo[0] && xs < A.hi[0];
+    // edge group of the lane (EDG): left of the segment for lanes 0-31, right of it for 32-63
+    const int64_t xe = min(max(lane < 32 ? xs0 - VX : xs0 + 64 * VX, (int64_t)0), n0 - VX);
 
     const C* wxt = wtab<C>(A, 0);
     const C* wyt = wtab<C>(A, 1);
@@ -411,13 +486,15 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     }
 
     const T* ucol = u + xl;
+    const T* ecol = u + xe;
     // rows outside the chunk's tap rows (the prefetch of its last steps) reload its last tap
     // row, already in cache, instead of reading rows this wave never uses
     const int ylo = max(y0 - H, 0);
     const int ylast = min(y1 + H - 1, n1 - 1);
-    auto load_row = [&](int row, C* dst) {
+    auto load_row = [&](int row, C* dst, C* edst) {
         row = min(max(row, ylo), ylast);
         VecIO<T, VX>::ld(ucol + (int64_t)row * s1, dst);
+        if constexpr (EDG) VecIO<T, VX>::ld(ecol + (int64_t)row * s1, edst);
     };
     const int nrows = y1 - y0;
     // The march over the chunk's rows, downwards (DIR = 1) or upwards (DIR = -1).  Ring slot
@@ -427,17 +504,22 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
         constexpr int DIR = decltype(dirc)::value;
         const int ystart = DIR > 0 ? y0 : y1 - 1;
         C q[R][VX];
+        C qe[EDG ? R : 1][1][VX];   // the row's edge group (EDG)
 #pragma unroll
-        for (int s = 0; s < R - 1; ++s) load_row(ystart + DIR * (s - H), q[s]);
+        for (int s = 0; s < R - 1; ++s) load_row(ystart + DIR * (s - H), q[s], qe[EDG ? s : 0][0]);
         for (int t0 = 0; t0 < nrows; t0 += R) {
             auto step = [&](auto jc) -> bool {
                 constexpr int j = decltype(jc)::value;
                 const int t = t0 + j;
                 if (t >= nrows) return false;
                 const int y = ystart + DIR * t;
-                load_row(y + DIR * (H + PF), q[(j + K - 1 + PF) % R]);
+                constexpr int sl = (j + K - 1 + PF) % R;
+                load_row(y + DIR * (H + PF), q[sl], qe[EDG ? sl : 0][0]);
                 C xw[NWIN];
-                x_window_dpp<T, VX, H>(q[(j + H) % R], xw);
+                if constexpr (EDG)
+                    x_window_edges<C, VX, H>(q[(j + H) % R], qe[(j + H) % R], xw);
+                else
+                    x_window_dpp<T, VX, H>(q[(j + H) % R], xw);
                 C wy[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) wy[k] = UNI ? wyu[k] : wyt[(int64_t)y * K + k];
@@ -678,72 +760,6 @@ __device__ inline double uniform_val<double>(double v)
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-// lane i <- lane i-1 (lane 0 keeps `fill`) / lane i <- lane i+1 (lane 63 keeps `fill`)
-__device__ inline float dpp_shr1_fill(float fill, float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
-                                                                  __builtin_bit_cast(int, v),
-                                                                  0x138, 0xf, 0xf, false));
-}
-__device__ inline float dpp_shl1_fill(float fill, float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
-                                                                  __builtin_bit_cast(int, v),
-                                                                  0x130, 0xf, 0xf, false));
-}
-__device__ inline double dpp2(double fill, double v, int ctrl_is_shl)
-{
-    const uint64_t f = __builtin_bit_cast(uint64_t, fill), b = __builtin_bit_cast(uint64_t, v);
-    int lo, hi;
-    if (ctrl_is_shl) {
-        lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, 0x130, 0xf, 0xf, false);
-        hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), 0x130,
-                                         0xf, 0xf, false);
-    } else {
-        lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, 0x138, 0xf, 0xf, false);
-        hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), 0x138,
-                                         0xf, 0xf, false);
-    }
-    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-__device__ inline double dpp_shr1_fill(double fill, double v) { return dpp2(fill, v, 0); }
-__device__ inline double dpp_shl1_fill(double fill, double v) { return dpp2(fill, v, 1); }
-
-// x window xw[e] = u(xs - H + e), e < VX + 2H, of a lane's VX points: its own values, its
-// neighbours' through DPP, and at the wave's ends the edge groups eg[g] (lane 0: group
-// -(g+1) of the segment, lane 63: group 64+g).
-template <typename C, int VX, int H>
-__device__ inline void x_window_edges(const C* own, const C (*eg)[VX], C* xw)
-{
-    constexpr int NE = (H + VX - 1) / VX;
-#pragma unroll
-    for (int v = 0; v < VX; ++v) xw[H + v] = own[v];
-    C l[VX], r[VX];
-#pragma unroll
-    for (int v = 0; v < VX; ++v) {
-        l[v] = own[v];
-        r[v] = own[v];
-    }
-#pragma unroll
-    for (int d = 1; d <= NE; ++d) {
-#pragma unroll
-        for (int v = 0; v < VX; ++v) {
-            const int el = H - d * VX + v;              // u(xs - d*VX + v)
-            const int er = H + VX + (d - 1) * VX + v;   // u(xs + d*VX + v)
-            // value v of the group at distance d lands in the window (and then so does every
-            // nearer one, so the shift chain of v runs up to the farthest such d)
-            if (el >= 0) {
-                l[v] = dpp_shr1_fill(eg[d - 1][v], l[v]);
-                xw[el] = l[v];
-            }
-            if (er < VX + 2 * H) {
-                r[v] = dpp_shl1_fill(eg[d - 1][v], r[v]);
-                xw[er] = r[v];
-            }
-        }
-    }
 }
 
 template <typename T, int VX, int K, bool UNI, int PF, int W, int VY, int ACC>
@@ -1005,15 +1021,24 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
     }
     const dim3 grid((unsigned)(A.nblocks + nshell));
     if (fp.kind == 2) {
-#define DOP_L2(UNI, PF)                                                                      \
-    hipLaunchKernelGGL((k_tiled2d<T, VX, K, UNI, PF, ACC>), grid, dim3(256), 0, s, A, p.dev,   \
+#define DOP_L2E(UNI, PF, EDG)                                                                \
+    hipLaunchKernelGGL((k_tiled2d<T, VX, K, UNI, PF, ACC, EDG>), grid, dim3(256), 0, s, A, p.dev, \
                        fp.shell, nshell, (T*)du, (const T*)u, accumulate)
+#define DOP_L2(UNI, PF)                                                                      \
+    do {                                                                                     \
+        if constexpr (K / 2 <= VX) {                                                         \
+            if (A.gen == 2) { DOP_L2E(UNI, PF, true); break; }                               \
+        }                                                                                    \
+        if (A.gen == 2) return hipErrorInvalidValue;                                         \
+        DOP_L2E(UNI, PF, false);                                                             \
+    } while (0)
         if (uni) {
             if (A.PF == 4) DOP_L2(true, 4); else DOP_L2(true, 2);
         } else {
             if (A.PF == 4) DOP_L2(false, 4); else DOP_L2(false, 2);
         }
 #undef DOP_L2
+#undef DOP_L2E
     } else {
         if constexpr (K <= 9 && VX <= 16 / (int)sizeof(T) && VX <= 4) {
 #define DOP_L3(UNI, W, VY)                                                                   \

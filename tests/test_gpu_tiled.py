@@ -148,23 +148,27 @@ def plan_with_variant(A, u, v):
         os.environ.pop("DOP_TILE_VARIANT", None)
 
 
-def n_variants(shape, dtype):
-    """Tile-shape candidates of the tiled path (dop_fast.cpp: fast_variant_count)."""
+def n_variants(shape, dtype, kh):
+    """Tile-shape candidates of the tiled path (dop_fast.cpp: fast_variant_count); kh is the
+    window's half-width."""
     n0 = shape[-1]
     vx = 2 if dtype == torch.float64 else 4
     vx = vx if n0 % vx == 0 else 1
     if len(shape) == 2:
         # chunk x prefetch x remap (16) x points per lane (2, when rows allow) x marches (2)
-        return 64 if vx > 1 and n0 % (2 * vx) == 0 else 32
+        # x segments (overlapping; edge-loaded when kh <= vx)
+        base = 64 if vx > 1 and n0 % (2 * vx) == 0 else 32
+        return base * (2 if kh <= vx else 1)
     # overlapping segments (40 / 20) and edge-loaded segments (32 / 16)
     return 72 if vx > 1 else 36
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
-@pytest.mark.parametrize("shape,kh", [((300, 260), 2), ((40, 72, 100), 3), ((33, 96), 5),
+@pytest.mark.parametrize("shape,kh", [((300, 260), 2), ((40, 1040), 2), ((50, 301), 1),
+                                      ((40, 72, 100), 3), ((33, 96), 5),
                                       ((21, 45, 532), 3), ((14, 30, 266), 4)])
 def test_every_tile_variant_bitwise(shape, kh, dtype):
-    nvar = n_variants(shape, dtype)
+    nvar = n_variants(shape, dtype, kh)
     A, u = build(shape, kh, dtype, seed=5, uniform=False, upwind=(kh == 3))
     ref = torch.empty_like(u)
     generic_of(A, u).apply(ref, u)
