@@ -107,41 +107,6 @@ __global__ __launch_bounds__(kBlock) void k_apply_line(const PlanDev* __restrict
 // loads (element-aligned; the padded inputs of the vector calculus have odd strides), uniform
 // weights read once per thread, and one vector store -- a quarter of the memory instructions,
 // which bound the one-point form.
-template <typename T>
-struct Quad;
-template <>
-struct Quad<float> {
-    typedef float V __attribute__((ext_vector_type(4), aligned(4)));
-    __device__ static void ld(const float* p, float* o)
-    {
-        const V v = *reinterpret_cast<const V*>(p);
-        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-    }
-    __device__ static void st(float* p, const double* v)
-    {
-        V o;
-        o.x = (float)v[0]; o.y = (float)v[1]; o.z = (float)v[2]; o.w = (float)v[3];
-        *reinterpret_cast<V*>(p) = o;
-    }
-};
-template <>
-struct Quad<double> {
-    typedef double V __attribute__((ext_vector_type(2), aligned(8)));
-    __device__ static void ld(const double* p, double* o)
-    {
-        const V a = *reinterpret_cast<const V*>(p);
-        const V b = *reinterpret_cast<const V*>(p + 2);
-        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
-    }
-    __device__ static void st(double* p, const double* v)
-    {
-        V a, b;
-        a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3];
-        *reinterpret_cast<V*>(p) = a;
-        *reinterpret_cast<V*>(p + 2) = b;
-    }
-};
-
 template <typename T, int ACC, int K>
 __global__ __launch_bounds__(kBlock) void k_apply_line4(const PlanDev* __restrict__ Pp,
                                                        T* __restrict__ du,

@@ -19,6 +19,43 @@ struct DevicePlan {
     int generic_only = 0;   // every point on k_apply_generic (no tiles, no single-axis kernel)
 };
 
+// Four consecutive values as one vector load / store (element-aligned: addresses need not be
+// 16-byte aligned).
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+    typedef float V __attribute__((ext_vector_type(4), aligned(4)));
+    __device__ static void ld(const float* p, float* o)
+    {
+        const V v = *reinterpret_cast<const V*>(p);
+        o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    }
+    __device__ static void st(float* p, const double* v)
+    {
+        V o;
+        o.x = (float)v[0]; o.y = (float)v[1]; o.z = (float)v[2]; o.w = (float)v[3];
+        *reinterpret_cast<V*>(p) = o;
+    }
+};
+template <>
+struct Quad<double> {
+    typedef double V __attribute__((ext_vector_type(2), aligned(8)));
+    __device__ static void ld(const double* p, double* o)
+    {
+        const V a = *reinterpret_cast<const V*>(p);
+        const V b = *reinterpret_cast<const V*>(p + 2);
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    }
+    __device__ static void st(double* p, const double* v)
+    {
+        V a, b;
+        a.x = v[0]; a.y = v[1]; b.x = v[2]; b.y = v[3];
+        *reinterpret_cast<V*>(p) = a;
+        *reinterpret_cast<V*>(p + 2) = b;
+    }
+};
+
 constexpr int kBlock = 256;        // threads per block
 constexpr int kChunkPts = 1024;    // points per chunk of a persistent launch
 

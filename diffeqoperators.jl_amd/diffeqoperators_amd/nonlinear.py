@@ -4,7 +4,8 @@
 
 (the Leibniz expansion of d^s/dx^s (p d^f q/dx^f)), with D^k = CenteredDifference(k,
 approx_order, dx, nknots) on the padded vectors p and q (length nknots + 2).  Every term of a
-point is evaluated by one thread of one libdop launch (dop_nonlinear_diffusion).
+point is evaluated in one libdop launch (dop_nonlinear_diffusion): on uniform grids one thread
+takes four points and reads their windows of p and q once; otherwise a thread per point.
 """
 from __future__ import annotations
 

@@ -47,3 +47,33 @@ def test_nonlinear_diffusion_errors():
         D.nonlinear_diffusion(1, 1, 2, q, q, 0.1, 11)
     with pytest.raises(TypeError):
         D.nonlinear_diffusion(1, 1, 2, q.cpu(), q.cpu(), 0.1, 10)
+
+
+@pytest.mark.parametrize("s,f,a", [(1, 1, 2), (2, 1, 4), (3, 0, 2), (2, 2, 6), (1, 0, 4), (4, 1, 2)])
+@pytest.mark.parametrize("n", [61, 4096, 4097])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_nonlinear_diffusion_four_point_form_bitwise(rng, s, f, a, n, dtype):
+    """Uniform grids take the four-point kernel (windows read once per group of points); it
+    must equal the per-point form bit for bit, at the ends and for lengths not a multiple of
+    four included."""
+    import os
+    from diffeqoperators_amd import nonlinear as _nonlinear
+    print(f"case: nld4 s={s} f={f} a={a} n={n} {dtype}", flush=True)
+    dx = 1.0 / (n + 1)
+    p = torch.from_numpy(rng.uniform(0.5, 1.5, n + 2)).to("cuda", dtype)
+    q = torch.from_numpy(rng.standard_normal(n + 2)).to("cuda", dtype)
+    outs = []
+    for generic in (False, True):
+        if generic:
+            os.environ["DOP_FORCE_GENERIC"] = "1"
+        _nonlinear._CACHE.clear()
+        try:
+            outs.append(D.nonlinear_diffusion(s, f, a, p, q, dx, n))
+        finally:
+            os.environ.pop("DOP_FORCE_GENERIC", None)
+            _nonlinear._CACHE.clear()
+    assert torch.equal(outs[0], outs[1])
+    ref = po.nonlinear_diffusion(s, f, a, p.double().cpu().numpy(), q.double().cpu().numpy(), dx, n)
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    got = outs[0].double().cpu().numpy()
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
