@@ -194,6 +194,26 @@ def bench_widened(steps=50, warmup=5):
     return out
 
 
+def bench_copy(steps=50, warmup=5, sizes_mib=(64, 128, 256, 512, 1024)):
+    """A plain device copy (torch copy_) of each size to a second buffer of that size: the rate
+    the memory system gives a pure stream.  Working sets that fit MI355X's 256 MB Infinity Cache
+    (a copy of up to 128 MiB) are served from it when repeated, as the 2-D 4096^2 applies are;
+    larger ones run at the HBM rate that bounds the 512^3 apply."""
+    import torch
+    out = {}
+    for mib in sizes_mib:
+        a = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        _, ev = time_applies(lambda: b.copy_(a), steps, warmup)
+        t = ev / steps
+        gbs = 2 * (mib << 20) / t / 1e9
+        out[f"{mib}MiB"] = dict(us=round(t * 1e6, 1), read_write_GBs=round(gbs, 1),
+                                frac=round(gbs / HBM_PEAK_GBS, 4))
+        del a, b
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline_laplacian2d(seconds):
     """The oracle (FP64 C restatement of the reference's per-line loops) on the same
     workload, one thread, on a bounded sample: whole applies until `seconds` elapse."""
@@ -690,6 +710,8 @@ def main():
         also["batched_rk4_1024"] = bench_batched(max(10, args.steps // 10))
         torch.cuda.empty_cache()
         also["served_advdiff"] = bench_served(args.requests)
+        torch.cuda.empty_cache()
+        also["copy_reference"] = bench_copy()
     print(json.dumps(line), flush=True)
 
 
