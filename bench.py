@@ -95,7 +95,8 @@ def bench_laplacian2d(steps, warmup):
     n = 4096
     A, u, _, _ = laplacian2d(n)
     plan = D.plans_for(A, u)[0]
-    variant = plan.autotune()
+    # a tile variant pinned by DOP_TILE_VARIANT (profiling runs) is kept
+    variant = plan.variant[0] if os.environ.get("DOP_TILE_VARIANT") else plan.autotune()
     du = torch.empty_like(u)
     wall, ev = time_applies(lambda: plan.apply(du, u), steps, warmup)
     pts = n * n
