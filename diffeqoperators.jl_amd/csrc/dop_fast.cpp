@@ -259,7 +259,7 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     fp->kind = P.ndim;
 }
 
-// Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} x XCD remap x
+// Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} ({4,8} edge-loaded) x XCD remap x
 // points per lane {widest 16-byte, twice that} x marches {all downwards, alternating} x
 // segments {overlapping, edge-loaded (when the x half-width is at most the points per lane)}; 3-D:
 // chunk depth {64,32} x block shape (waves, rows per wave) {(8,1), (16,1), (8,2), (16,2),
@@ -317,7 +317,8 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
     if (fp.kind == 2) {
         static const int chunks[4] = {32, 16, 64, 8};
         A.chunk = chunks[(v >> 1) & 3];
-        A.PF = ((v >> 3) & 1) ? 4 : 2;
+        // edge-loaded rings carry H edge values per row, so they afford deeper prefetch
+        A.PF = A.gen == 2 ? (((v >> 3) & 1) ? 8 : 4) : (((v >> 3) & 1) ? 4 : 2);
         // bit 4: twice the points per lane (when rows allow), next bit: alternating marches
         const bool wide = wide_ok(fp);
         A.VX = (wide && ((v >> 4) & 1)) ? 2 * fp.VX : fp.VX;

@@ -417,6 +417,40 @@ __device__ inline void x_window_edges(const C* own, const C (*eg)[VX], C* xw)
     }
 }
 
+// H consecutive values from an element-aligned address, in the compute type
+template <typename T, int H>
+struct EdgeIO {
+    using C = typename Store<T>::C;
+    __device__ static void ld(const T* p, C* o)
+    {
+        if constexpr (!std::is_same<T, bf16>::value && (H == 2 || H == 4)) {
+            typedef T V __attribute__((ext_vector_type(H), aligned(sizeof(T))));
+            const V v = *reinterpret_cast<const V*>(p);
+#pragma unroll
+            for (int h = 0; h < H; ++h) o[h] = v[h];
+        } else {
+#pragma unroll
+            for (int h = 0; h < H; ++h) o[h] = Store<T>::ld(p, h);
+        }
+    }
+};
+
+// x window of the edge-loaded 2-D form (H <= VX): own values, the neighbours' through DPP, and
+// at the wave's ends the H values beyond the segment, e[] (lane 0: u(xs0 - H + j), lane 63:
+// u(xs0 + 64*VX + j)).  xw[e] = u(xs - H + e), e < VX + 2H.
+template <typename C, int VX, int H>
+__device__ inline void x_window_edgeh(const C* own, const C* e, C* xw)
+{
+    static_assert(H <= VX, "the window reaches one lane on each side");
+#pragma unroll
+    for (int v = 0; v < VX; ++v) xw[H + v] = own[v];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        xw[j] = dpp_shr1_fill(e[j], own[VX - H + j]);
+        xw[H + VX + j] = dpp_shl1_fill(e[j], own[j]);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // 2-D: a wave loads an aligned 64*VX-point row segment and marches a chunk of rows with a
 // register ring of K + PF rows (PF rows prefetched ahead).  Overlapping form (EDG false): the
@@ -424,9 +458,9 @@ __device__ inline void x_window_edges(const C* own, const C (*eg)[VX], C* xw)
 // 2*PADL, the x window comes from neighbouring lanes through DPP, and only the middle lanes
 // store.  Edge-loaded form (EDG true, H <= VX): segments start at multiples of 64*VX points
 // and do not overlap (every row read is whole 128-byte lines and no point is read twice by
-// neighbouring segments); each lane also loads one edge group per row -- lanes 0-31 the
-// group left of the segment, lanes 32-63 the one right of it, two addresses per wave -- and
-// lanes 0 and 63 feed theirs into the DPP window.  Loads are clamped into the array and issued
+// neighbouring segments); each lane also loads the H values beside the segment on one side per
+// row -- lanes 0-31 those left of it, lanes 32-63 those right of it, two addresses per wave --
+// and lanes 0 and 63 feed theirs into the DPP window.  Loads are clamped into the array and issued
 // on every path, stores are one predicated vector store, so the in-order load counter waits
 // stay precise and the ring stays in flight.
 template <typename T, int VX, int K, bool UNI, int PF, int ACC, bool EDG = false>
@@ -466,8 +500,8 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     // the box are stored and their windows stay within H of the box, inside the array.
     const int64_t xl = min(max(xs, (int64_t)0), n0 - VX);
     const bool st_ok = (EDG || (lane >= LEDGE && lane < 64 - LEDGE)) && xs >= A.lo[0] && xs < A.hi[0];
-    // edge group of the lane (EDG): left of the segment for lanes 0-31, right of it for 32-63
-    const int64_t xe = min(max(lane < 32 ? xs0 - VX : xs0 + 64 * VX, (int64_t)0), n0 - VX);
+    // the lane's H edge values (EDG): left of the segment for lanes 0-31, right of it for 32-63
+    const int64_t xe = min(max(lane < 32 ? xs0 - H : xs0 + 64 * VX, (int64_t)0), n0 - H);
 
     const C* wxt = wtab<C>(A, 0);
     const C* wyt = wtab<C>(A, 1);
@@ -494,7 +528,7 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
     auto load_row = [&](int row, C* dst, C* edst) {
         row = min(max(row, ylo), ylast);
         VecIO<T, VX>::ld(ucol + (int64_t)row * s1, dst);
-        if constexpr (EDG) VecIO<T, VX>::ld(ecol + (int64_t)row * s1, edst);
+        if constexpr (EDG) EdgeIO<T, H>::ld(ecol + (int64_t)row * s1, edst);
     };
     const int nrows = y1 - y0;
     // The march over the chunk's rows, downwards (DIR = 1) or upwards (DIR = -1).  Ring slot
@@ -504,9 +538,9 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
         constexpr int DIR = decltype(dirc)::value;
         const int ystart = DIR > 0 ? y0 : y1 - 1;
         C q[R][VX];
-        C qe[EDG ? R : 1][1][VX];   // the row's edge group (EDG)
+        C qe[EDG ? R : 1][H];   // the row's edge values (EDG)
 #pragma unroll
-        for (int s = 0; s < R - 1; ++s) load_row(ystart + DIR * (s - H), q[s], qe[EDG ? s : 0][0]);
+        for (int s = 0; s < R - 1; ++s) load_row(ystart + DIR * (s - H), q[s], qe[EDG ? s : 0]);
         for (int t0 = 0; t0 < nrows; t0 += R) {
             auto step = [&](auto jc) -> bool {
                 constexpr int j = decltype(jc)::value;
@@ -514,10 +548,10 @@ __global__ __launch_bounds__(256) void k_tiled2d(const FastArgs A, const PlanDev
                 if (t >= nrows) return false;
                 const int y = ystart + DIR * t;
                 constexpr int sl = (j + K - 1 + PF) % R;
-                load_row(y + DIR * (H + PF), q[sl], qe[EDG ? sl : 0][0]);
+                load_row(y + DIR * (H + PF), q[sl], qe[EDG ? sl : 0]);
                 C xw[NWIN];
                 if constexpr (EDG)
-                    x_window_edges<C, VX, H>(q[(j + H) % R], qe[(j + H) % R], xw);
+                    x_window_edgeh<C, VX, H>(q[(j + H) % R], qe[(j + H) % R], xw);
                 else
                     x_window_dpp<T, VX, H>(q[(j + H) % R], xw);
                 C wy[K];
@@ -1024,19 +1058,18 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
 #define DOP_L2E(UNI, PF, EDG)                                                                \
     hipLaunchKernelGGL((k_tiled2d<T, VX, K, UNI, PF, ACC, EDG>), grid, dim3(256), 0, s, A, p.dev, \
                        fp.shell, nshell, (T*)du, (const T*)u, accumulate)
-#define DOP_L2(UNI, PF)                                                                      \
+#define DOP_L2(UNI)                                                                          \
     do {                                                                                     \
-        if constexpr (K / 2 <= VX) {                                                         \
-            if (A.gen == 2) { DOP_L2E(UNI, PF, true); break; }                               \
+        if (A.gen == 2) {                                                                    \
+            if constexpr (K / 2 <= VX) {                                                     \
+                if (A.PF == 8) DOP_L2E(UNI, 8, true); else DOP_L2E(UNI, 4, true);            \
+                break;                                                                       \
+            }                                                                                \
+            return hipErrorInvalidValue;                                                     \
         }                                                                                    \
-        if (A.gen == 2) return hipErrorInvalidValue;                                         \
-        DOP_L2E(UNI, PF, false);                                                             \
+        if (A.PF == 4) DOP_L2E(UNI, 4, false); else DOP_L2E(UNI, 2, false);                  \
     } while (0)
-        if (uni) {
-            if (A.PF == 4) DOP_L2(true, 4); else DOP_L2(true, 2);
-        } else {
-            if (A.PF == 4) DOP_L2(false, 4); else DOP_L2(false, 2);
-        }
+        if (uni) DOP_L2(true); else DOP_L2(false);
 #undef DOP_L2
 #undef DOP_L2E
     } else {
