@@ -304,8 +304,9 @@ int ts_setup(dop_tsit5* I, double scale, double t0, double t1, double dt0, doubl
     int rc = DOP_OK;
     // k1 = L Q u0 (unscaled; f = scale * k).  Without dt0: the starting step size of Hairer,
     // Norsett & Wanner (Solving ODEs I, II.4) from the norms of u0, f(u0) and f(u0 + h0 f(u0)).
-    if (launch_apply(I->plan->d, I->a.k[0], I->a.u, 0, -1, 0, s) != hipSuccess)
-        return tfail(DOP_EHIP, "tsit5: initial apply failed");
+    // Through dop_apply: an untuned plan of at least 2^20 points picks its tile variant here
+    // (on k1's buffer and u0, before the attempt is captured), as on its first apply elsewhere.
+    if ((rc = dop_apply(I->plan, I->a.k[0], I->a.u, 0, s)) != DOP_OK) return rc;
     double dt = dt0;
     if (!(dt > 0.0)) {
         const double d0 = ts_norm2<T>(I, I->a.u, nullptr, 0.0, 1.0, atol, rtol, s, &rc);
