@@ -222,6 +222,7 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     }
     // points per lane: 16-byte lane loads when rows stay aligned
     const int vx_native = (P.storage == kF64) ? 2 : 4;
+    fp->f64 = P.storage == kF64;
     fp->VX = (P.n[0] % vx_native == 0) ? vx_native : 1;
     if (P.ndim >= 2) {
         // lanes store whole VX groups: the box's x edges are aligned to the widest VX of the
@@ -259,7 +260,7 @@ void build_fast(const HostPlan& H, FastPlan* fp)
     fp->kind = P.ndim;
 }
 
-// Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} ({4,8} edge-loaded) x XCD remap x
+// Candidates (autotuner).  2-D: chunk height {32,16,64,8} x prefetch {2,4} ({4,8}, FP64 {8,16} edge-loaded) x XCD remap x
 // points per lane {widest 16-byte, twice that} x marches {all downwards, alternating} x
 // segments {overlapping, edge-loaded (when the x half-width is at most the points per lane)}; 3-D:
 // chunk depth {64,32} x block shape (waves, rows per wave) {(8,1), (16,1), (8,2), (16,2),
@@ -317,8 +318,11 @@ bool fast_variant(const FastPlan& fp, int v, FastArgs* out)
     if (fp.kind == 2) {
         static const int chunks[4] = {32, 16, 64, 8};
         A.chunk = chunks[(v >> 1) & 3];
-        // edge-loaded rings carry H edge values per row, so they afford deeper prefetch
-        A.PF = A.gen == 2 ? (((v >> 3) & 1) ? 8 : 4) : (((v >> 3) & 1) ? 4 : 2);
+        // edge-loaded rings carry H edge values per row, so they afford deeper prefetch (FP64
+        // rows of 2 points per lane: 8 or 16 rows; 4 or 8 otherwise)
+        const bool f64 = fp.f64 != 0;
+        A.PF = A.gen == 2 ? (((v >> 3) & 1) ? (f64 ? 16 : 8) : (f64 ? 8 : 4))
+                          : (((v >> 3) & 1) ? 4 : 2);
         // bit 4: twice the points per lane (when rows allow), next bit: alternating marches
         const bool wide = wide_ok(fp);
         A.VX = (wide && ((v >> 4) & 1)) ? 2 * fp.VX : fp.VX;

@@ -97,6 +97,7 @@ struct FastPlan {
     int kind = 0;          // 0: generic only, 1: 1-D window tables only (batched tiles),
                            // 2: 2-D tiled, 3: 3-D tiled
     int VX = 1;            // widest points-per-lane the row alignment allows
+    int f64 = 0;           // FP64 storage
     int kahan = 0;         // Dot2-compensated f32 accumulation: batched tiles only
     FastArgs args{};
     ShellArgs shell{};

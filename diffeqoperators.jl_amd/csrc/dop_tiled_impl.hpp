@@ -1062,7 +1062,11 @@ hipError_t launch_interior_tiles(const DevicePlan& p, const FastArgs& A, void* d
     do {                                                                                     \
         if (A.gen == 2) {                                                                    \
             if constexpr (K / 2 <= VX) {                                                     \
-                if (A.PF == 8) DOP_L2E(UNI, 8, true); else DOP_L2E(UNI, 4, true);            \
+                if constexpr (sizeof(T) == 8) {                                              \
+                    if (A.PF == 16) DOP_L2E(UNI, 16, true); else DOP_L2E(UNI, 8, true);      \
+                } else {                                                                     \
+                    if (A.PF == 8) DOP_L2E(UNI, 8, true); else DOP_L2E(UNI, 4, true);        \
+                }                                                                            \
                 break;                                                                       \
             }                                                                                \
             return hipErrorInvalidValue;                                                     \
